@@ -1,0 +1,241 @@
+"""Benchmark of the step loop on one MI355X.
+
+Headline (`value`): particle-steps/s of the batched ensemble -- 512 independent
+replicas (256 of the two-species LJ mixture, 256 of the wall + reaction model,
+N per replica log-uniform over 2,048..32,768, sum N = 5,789,872) stepped
+together with FP32 pair forces and FP64 positions, hipGraph-captured.  Beside it:
+
+* `plain`: the single-trajectory FP64 path, one replica at a time, on a
+  stratified subset of the same replicas, and the ratio batched / plain;
+* `roofline`: the fused force + integrate + bin kernel's average duration from
+  HIP events on the engine's stream during an eager pass over the same state,
+  against its algorithmic HBM bytes (48 B per particle + 4 B per cell);
+* `cpu_baseline`: the CPU oracle in reference-faithful mode (Verlet list on the
+  reference's grid, FP64, one core) on a bounded sample of the same replicas.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import math
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from pyrdsim_amd import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FORCE_BYTES_PER_PARTICLE = 48   # read pos 16 + type 4 + pid 4; write pos 16 + cell 4 + rank 4
+FORCE_BYTES_PER_CELL = 4        # cell starts
+
+
+def _make_replica_state(args):
+    kind, n, k_a, k_d, seed, ic_seed = args
+    if kind == 'kalj':
+        s = workloads.kalj(n=n, seed=ic_seed)
+    else:
+        s = workloads.wall(n=n, seed=ic_seed, k_a=k_a, k_d=k_d)
+    return s['x'], s['types']
+
+
+def build_ensemble(n_rep, workers):
+    plan = workloads.ensemble_plan(n_rep=n_rep)
+    jobs = [(p['kind'], p['n'], p['k_a'], p['k_d'], p['seed'], 1005 * 1000 + r)
+            for r, p in enumerate(plan)]
+    ctx = mp.get_context('fork')
+    with ctx.Pool(workers) as pool:
+        states = pool.map(_make_replica_state, jobs, chunksize=4)
+    return plan, states
+
+
+def engine_pieces(plan):
+    from pyrdsim_amd.engine import build_model, make_replica
+    mk = workloads.kalj_model()
+    mw = workloads.wall_model()
+    kalj_model, _ = build_model(mk['interactions'], mk['reactions'], 2)
+    wall_model, _ = build_model(mw['interactions'], mw['reactions'], 2)
+    L_k = lambda n: workloads.square_box(n, 0.6)
+    L_w = lambda n: workloads.square_box(n, 0.4)
+    reps = []
+    for p in plan:
+        if p['kind'] == 'kalj':
+            reps.append(make_replica(0, p['n'], L_k(p['n']), [1, 1], 5e-5, [1.0, 1.0], [],
+                                     p['seed']))
+        else:
+            reps.append(make_replica(1, p['n'], L_w(p['n']), [1, -1], 1e-4, [1.0, 1.0],
+                                     [p['k_a'], p['k_d']], p['seed']))
+    return [kalj_model, wall_model], reps
+
+
+def time_engine(eng, steps, warmup):
+    import torch
+    eng.run(warmup)
+    eng.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.run(steps)
+    eng.sync()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(plan, states, budget_s):
+    """The oracle, reference-faithful (Verlet list, rs = 2 rc, ratio 2, refresh
+    every 10 steps, FP64, one core), on the smallest replica of each model."""
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    from oracle import Oracle
+    picks = []
+    for kind in ('kalj', 'wall'):
+        idx = [r for r, p in enumerate(plan) if p['kind'] == kind]
+        picks.append(min(idx, key=lambda r: plan[r]['n']))
+    done = 0
+    t_total = 0.0
+    desc = []
+    for r in picks:
+        p = plan[r]
+        x, t = states[r]
+        if p['kind'] == 'kalj':
+            m = workloads.kalj_model()
+            L, per, dt = workloads.square_box(p['n'], 0.6), [1, 1], 5e-5
+        else:
+            m = workloads.wall_model(p['k_a'], p['k_d'])
+            L, per, dt = workloads.square_box(p['n'], 0.4), [1, -1], 1e-4
+        o = Oracle(x, t, L, per, dt, m['tau_by_tauD'], m['interactions'], m['reactions'],
+                   seed=p['seed'], rng='mt19937', nl='faithful', ratio=2, nl_freq=10)
+        o.init()
+        steps = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < budget_s / len(picks):
+            o.run(steps + 1, 10)
+            steps += 10
+        el = time.perf_counter() - t0
+        done += p['n'] * steps
+        t_total += el
+        desc.append('%s N=%d x %d steps' % (p['kind'], p['n'], steps))
+    return done / t_total, '; '.join(desc) + ' (%.1f s)' % t_total
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=500)
+    ap.add_argument('--warmup', type=int, default=50)
+    ap.add_argument('--replicas', type=int, default=512)
+    ap.add_argument('--plain-subset', type=int, default=32)
+    ap.add_argument('--plain-steps', type=int, default=500)
+    ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    ap.add_argument('--graph-steps', type=int, default=50)
+    ap.add_argument('--tile-particles', type=int, default=0)
+    ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path '
+              '(--gpus %d requested)' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    plan, states = build_ensemble(args.replicas, min(16, os.cpu_count() or 1))
+    print('# built %d replica states in %.1f s' % (len(plan), time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+
+    import torch
+    from pyrdsim_amd.engine import Engine
+    models, reps = engine_pieces(plan)
+    n_tot = sum(p['n'] for p in plan)
+    x0 = np.concatenate([s[0] for s in states])
+    t0s = np.concatenate([s[1] for s in states])
+
+    eng = Engine(models, reps, precision='mixed', graph_steps=args.graph_steps,
+                 tile_particles=args.tile_particles)
+    eng.set_state(x0, t0s)
+    secs = time_engine(eng, args.steps, args.warmup)
+    value = n_tot * args.steps / secs
+    print('# batched: %.3e particle-steps/s (%.3f ms/step)' % (value, 1e3 * secs / args.steps),
+          file=sys.stderr, flush=True)
+
+    kt = eng.time_kernels(20)
+    eng.sync()
+    _, n_cells, n_tiles = eng.sizes()
+    force_ms = kt['force'][0]
+    force_bytes = FORCE_BYTES_PER_PARTICLE * n_tot + FORCE_BYTES_PER_CELL * n_cells
+    achieved = force_bytes / (force_ms * 1e-3) / 1e9
+    roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel='k_force (pair forces + Euler-Maruyama + binning)',
+                    algorithmic_bytes=int(force_bytes),
+                    avg_ms={k: round(v[0], 5) for k, v in kt.items() if v[1]})
+    del eng
+    torch.cuda.synchronize()
+
+    plain = None
+    if not args.no_plain:
+        order = np.argsort([p['n'] for p in plan], kind='stable')
+        k = min(args.plain_subset, len(plan))
+        # stratified over size, alternating the two models
+        pick = [int(order[int((i + 0.5) * len(order) / k)]) for i in range(k)]
+        work = 0.0
+        tsum = 0.0
+        for r in pick:
+            e1 = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64',
+                        graph_steps=args.graph_steps)
+            e1.set_state(states[r][0], states[r][1])
+            dt = time_engine(e1, args.plain_steps, 20)
+            work += plan[r]['n'] * args.plain_steps
+            tsum += dt
+            e1.close()
+        plain = dict(value=work / tsum, unit='particle-steps/s', replicas=k,
+                     steps=args.plain_steps, precision='fp64', graph_steps=args.graph_steps)
+        print('# plain: %.3e particle-steps/s' % plain['value'], file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        v, desc = cpu_baseline(plan, states, args.cpu_seconds)
+        cpu = dict(value=v, unit='particle-steps/s', cores=1, kind='port',
+                   sample='oracle, reference-faithful FP64: ' + desc)
+
+    out = {
+        'metric': 'particle-steps/s (batched vs plain, 1 MI355X) at RDF error <=1%; % HBM roofline',
+        'value': value,
+        'unit': 'particle-steps/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': 1e3 * secs / args.steps,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 forces + f64 positions',
+        'data': 'synthetic (seeded random initial states)',
+        'config': {
+            'workload': 'ensemble: %d replicas (half LJ mixture rho=0.6 dt=5e-5, half '
+                        'wall+reactions rho=0.4 dt=1e-4), N log-uniform 2048..32768, '
+                        'sum N=%d' % (len(plan), n_tot),
+            'precision': 'mixed', 'graph_steps': args.graph_steps,
+            'cells': int(n_cells), 'tiles': int(n_tiles)},
+        'roofline': roofline,
+        'plain': plain,
+        'batched_vs_plain': (value / plain['value']) if plain else None,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+
+
+def _with_model0(rep):
+    import ctypes
+    from pyrdsim_amd import _lib as L
+    r = L.Replica()
+    ctypes.pointer(r)[0] = rep
+    r.model = 0
+    return r
+
+
+if __name__ == '__main__':
+    main()

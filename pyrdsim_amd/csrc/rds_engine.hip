@@ -1,0 +1,1659 @@
+// MI355X (gfx950) engine for PyRDSim's overdamped-Langevin time-step loop.
+//
+// What one step does (the reference's integration_step, pyrdsim.pyx:2296-2322):
+//   x_{n+1}   = BC( x_n + fpre*F(x_n, tau_n) + npre*xi )     update_positions,
+//                                                            apply_boundary_conditions
+//   tau_{n+1} = R( x_{n+1}, tau_n )                          reaction_step
+//   F_{n+1}   = F( x_{n+1}, tau_{n+1} )                      update_drifts
+//
+// How it runs here.  Particles live in HBM sorted by cell ("slots"), FP64
+// positions as double2, int32 type / pid / molecule columns.  Many independent
+// systems ("replicas") are concatenated: replica r owns slots
+// [pbase_r, pbase_r + N_r) and cells [cbase_r, cbase_r + lx_r*ly_r).  Per step:
+//   k_bonds    (molecular models only) harmonic bond forces per slot
+//   k_force    one wavefront per tile (a strip of W cells of one cell row): the
+//              3 x (W+2) neighbour cells are staged in LDS with their periodic
+//              images already shifted; each lane owns one home particle and
+//              loops over its 3x3 stencil in LDS; the force is consumed at once
+//              by the Euler-Maruyama update, the boundary conditions and the
+//              binning of the new position (atomic per-cell histogram + rank)
+//   k_scan     one workgroup per replica: LDS prefix scan of the histogram into
+//              cell starts, histogram cleared for the next step, step counter
+//   k_scatter  counting-sort reorder of the new state into the other buffer
+//   k_react    (models with reactions) neighbour-dependent rates on the same
+//              staged tiles, decision with u1/u2, new type
+// Positions, the minimum-image displacement and the integrator are FP64; the
+// pair-force arithmetic is FP32 (RDS_MIXED) or FP64 (RDS_FP64).
+//
+// Random numbers: Philox4x32-10 (Salmon et al., SC'11) keyed by the replica
+// seed, counter (pid, step, stream); Box-Muller in FP32 for the two normals of
+// a particle, two 53-bit uniforms for the reactions; or injected buffers.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rds.h"
+
+#define MAXT RDS_MAX_TYPES
+#define MAXTERMS RDS_MAX_TERMS
+#define MAXWALLS RDS_MAX_WALLS
+#define MAXRX RDS_MAX_REACTIONS
+#define MAXW 60        // cells per tile strip
+#define CAP 384        // staged particles per LDS round
+#define FBLOCK 64      // threads of a force / reaction workgroup (one wave)
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(RDS_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device tables
+// ---------------------------------------------------------------------------
+
+struct DTerm {
+  int kind, n, pw, odd;  // pw: number of factors of the LJ power, odd: n odd
+  double u0, r0, rc2, a_inv, rin, c_lj, r0sq, slope;
+  float fu0, fr0, frc2, fa_inv, frin, fc_lj, fr0sq, fslope;
+};
+
+struct DWall {
+  int type, dir, pad0, pad1;
+  double u0, r0, a_inv;
+};
+
+struct DRx {
+  int kind, reactant, product, catalyst, inh, inh2, pad0, pad1;
+  double r0, rate_decrease;
+};
+
+struct DModel {
+  int nt, nterms, nwalls, nrx;
+  int nbr_rx;      // any catalysed / inhibited reaction
+  int pad0, pad1, pad2;
+  int pfirst[MAXT * MAXT];
+  int pcount[MAXT * MAXT];
+  DTerm terms[MAXTERMS];
+  DWall walls[MAXWALLS];
+  DRx rx[MAXRX];
+};
+
+struct DRep {
+  double L[2], cl[2];
+  int lx, ly, cbase, pbase;
+  int n, model, px, py;   // periodic axes (boundary conditions)
+  unsigned seed_lo, seed_hi;
+  int sx, sy;             // periodic images in the cell stencil
+  double fpre[MAXT], npre[MAXT], rate[MAXRX];
+  double dt;
+};
+
+struct DStatus {
+  unsigned long long err_key;  // (step << 32) | (pid*2 + axis), min over errors
+  double err_val;
+  int bounds, stream;
+  unsigned long long events;
+};
+
+// Everything a kernel of one step needs.  Passed by value.
+struct StepArgs {
+  const DRep* rep;
+  const DModel* models;
+  const int4* tiles;  // (replica, cy, cx0, w)
+  int n_tiles;
+  int n_total;
+  const int* cstart;  // [C_total + 1]
+  // current sorted state
+  const double2* pos;
+  const int* type;
+  const int* pid;
+  const int* mol;
+  // next-step outputs of k_force
+  double2* upos;
+  int* ucell;
+  int* urank;
+  int* count;
+  // bonds
+  const double2* fbond;
+  // force-only / count outputs (caller order)
+  double2* fout;
+  int* iout;
+  double rprobe2;
+  // random numbers
+  const long long* step;
+  const double* inj_n;
+  const double* inj_u1;
+  const double* inj_u2;
+  long long inj_first, inj_count;
+  int rng, prob_mode;
+  double include_rf;
+  DStatus* status;
+  // reaction output
+  int* type_out;
+};
+
+// ---------------------------------------------------------------------------
+// random numbers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, unsigned k0, unsigned k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    unsigned hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    unsigned hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Two standard normals of particle `pid` (index within its replica) at
+// reference step `step`.
+__device__ __forceinline__ void philox_normals(const DRep& R, int pid, long long step,
+                                               float& n0, float& n1) {
+  uint4 w = philox4x32_10(make_uint4((unsigned)pid, (unsigned)step,
+                                     (unsigned)((unsigned long long)step >> 32), 0u),
+                          R.seed_lo, R.seed_hi);
+  float ua = (float)((w.x >> 9) * 2u + 1u) * 0x1.0p-24f;  // odd / 2^24, in (0,1)
+  float ub = (float)(w.y >> 8) * 0x1.0p-24f;              // [0,1)
+  float rr = sqrtf(-2.0f * logf(ua));
+  float s, c;
+  sincospif(2.0f * ub, &s, &c);
+  n0 = rr * c;
+  n1 = rr * s;
+}
+
+// The two reaction uniforms u1 (decision) and u2 (choice), 53-bit.
+__device__ __forceinline__ void philox_uniforms(const DRep& R, int pid, long long step,
+                                                double& u1, double& u2) {
+  uint4 w = philox4x32_10(make_uint4((unsigned)pid, (unsigned)step,
+                                     (unsigned)((unsigned long long)step >> 32), 1u),
+                          R.seed_lo, R.seed_hi);
+  unsigned long long a = ((unsigned long long)w.y << 32) | w.x;
+  unsigned long long b = ((unsigned long long)w.w << 32) | w.z;
+  u1 = (double)(a >> 11) * 0x1.0p-53;
+  u2 = (double)(b >> 11) * 0x1.0p-53;
+}
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+// Cell coordinate floor(x / cl) (the reference's x // cl, pyrdsim.pyx:1497),
+// clamped into [0, l): a periodic wrap may return exactly L, and a coordinate
+// that left the box (reported separately) must still index a valid cell.
+__device__ __forceinline__ int cell_coord(double x, double cl, int l) {
+  double q = floor(x / cl);
+  if (!(q >= 0.0)) return 0;
+  if (q >= (double)l) return l - 1;
+  return (int)q;
+}
+
+// Block -> tile map that gives each XCD a contiguous run of tiles (workgroups
+// are dealt to the 8 XCDs round-robin; neighbouring tiles share halo rows, which
+// should then meet in the same L2).
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int X = 8;
+  int x = b % X, k = b / X;
+  int q = n / X, r = n % X;
+  return x * q + min(x, r) + k;
+}
+
+template <typename Real>
+struct Real2 {
+  Real x, y;
+};
+
+// ---------------------------------------------------------------------------
+// tile staging shared by the force and reaction kernels
+// ---------------------------------------------------------------------------
+
+struct TileShared {
+  int off[3][MAXW + 4];   // staged offset of cell k of row r (prefix, k = 0..W+2)
+  int seg_slot[9];        // up to 3 contiguous slot segments per row
+  int seg_n[9];
+  int seg_off[9];         // staged offset of the segment
+  double seg_sx[9], seg_sy[9];
+  int nseg;
+  int total;
+  int home_begin, home_end;
+  int row_base[3];
+  int mid_slot1, mid_off1; // home row: first slot and staged offset of the
+                           // contiguous in-range segment
+};
+
+// Computes the staged layout of a tile: for the three rows cy-1, cy, cy+1 the
+// cells cx0-1 .. cx0+W (periodic images shifted by -L / +L).
+__device__ void tile_layout(const DRep& R, const int* __restrict__ cstart, int cy, int cx0,
+                            int W, TileShared& sh) {
+  const int lane = threadIdx.x;
+  const int lx = R.lx, ly = R.ly;
+  // per-cell counts
+  for (int r = 0; r < 3; ++r) {
+    int y = cy + r - 1;
+    bool rowok = true;
+    if (y < 0) { if (R.sy) y += ly; else rowok = false; }
+    if (y >= ly) { if (R.sy) y -= ly; else rowok = false; }
+    for (int k = lane; k < W + 2; k += FBLOCK) {
+      int x = cx0 - 1 + k;
+      bool ok = rowok;
+      if (x < 0) { if (R.sx) x += lx; else ok = false; }
+      if (x >= lx) { if (R.sx) x -= lx; else ok = false; }
+      int cnt = 0;
+      if (ok) {
+        int c = R.cbase + y * lx + x;
+        cnt = cstart[c + 1] - cstart[c];
+      }
+      sh.off[r][k + 1] = cnt;
+    }
+  }
+  __syncthreads();
+  if (lane < 3) {
+    int r = lane;
+    int acc = 0;
+    sh.off[r][0] = 0;
+    for (int k = 1; k <= W + 2; ++k) {
+      acc += sh.off[r][k];
+      sh.off[r][k] = acc;
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    int base = 0, ns = 0;
+    for (int r = 0; r < 3; ++r) {
+      sh.row_base[r] = base;
+      int y = cy + r - 1;
+      double sy = 0.0;
+      bool rowok = true;
+      if (y < 0) { if (R.sy) { y += ly; sy = -R.L[1]; } else rowok = false; }
+      if (y >= ly) { if (R.sy) { y -= ly; sy = R.L[1]; } else rowok = false; }
+      if (rowok) {
+        int rowc = R.cbase + y * lx;
+        // left periodic image
+        if (cx0 - 1 < 0 && R.sx) {
+          int c = rowc + lx - 1;
+          sh.seg_slot[ns] = cstart[c];
+          sh.seg_n[ns] = cstart[c + 1] - cstart[c];
+          sh.seg_off[ns] = base + sh.off[r][0];
+          sh.seg_sx[ns] = -R.L[0];
+          sh.seg_sy[ns] = sy;
+          ++ns;
+        }
+        int xa = max(cx0 - 1, 0), xb = min(cx0 + W, lx - 1);
+        int ka = xa - (cx0 - 1);
+        sh.seg_slot[ns] = cstart[rowc + xa];
+        sh.seg_n[ns] = cstart[rowc + xb + 1] - cstart[rowc + xa];
+        sh.seg_off[ns] = base + sh.off[r][ka];
+        sh.seg_sx[ns] = 0.0;
+        sh.seg_sy[ns] = sy;
+        if (r == 1) {
+          sh.mid_slot1 = sh.seg_slot[ns];
+          sh.mid_off1 = sh.seg_off[ns];
+        }
+        ++ns;
+        if (cx0 + W >= lx && R.sx) {
+          int c = rowc;
+          sh.seg_slot[ns] = cstart[c];
+          sh.seg_n[ns] = cstart[c + 1] - cstart[c];
+          sh.seg_off[ns] = base + sh.off[r][W + 1];
+          sh.seg_sx[ns] = R.L[0];
+          sh.seg_sy[ns] = sy;
+          ++ns;
+        }
+      }
+      base += sh.off[r][W + 2];
+    }
+    sh.nseg = ns;
+    sh.total = base;
+    int hc = R.cbase + cy * lx + cx0;
+    sh.home_begin = cstart[hc];
+    sh.home_end = cstart[hc + W];
+  }
+  __syncthreads();
+}
+
+// Stages staged indices [p0, p0 + CAP) into LDS.
+template <bool MOL>
+__device__ void tile_stage(const TileShared& sh, int p0, const double2* __restrict__ pos,
+                           const int* __restrict__ type, const int* __restrict__ mol,
+                           double2* s_pos, int* s_ty, int* s_mol) {
+  const int lane = threadIdx.x;
+  const int p1 = p0 + CAP;
+  for (int g = 0; g < sh.nseg; ++g) {
+    int so = sh.seg_off[g], sn = sh.seg_n[g];
+    int a = max(so, p0), b = min(so + sn, p1);
+    double sx = sh.seg_sx[g], sy = sh.seg_sy[g];
+    int sl = sh.seg_slot[g] - so;
+    for (int p = a + lane; p < b; p += FBLOCK) {
+      double2 q = pos[sl + p];
+      q.x = q.x + sx;
+      q.y = q.y + sy;
+      s_pos[p - p0] = q;
+      s_ty[p - p0] = type[sl + p];
+      if (MOL) s_mol[p - p0] = mol[sl + p];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pair laws (pyrdsim.pyx:1068-1130).  All return c with F_i += c * (x_j - x_i),
+// i.e. c = f(r) / r where f = dU/dr is what the reference's closures return.
+// ---------------------------------------------------------------------------
+
+template <typename Real>
+__device__ __forceinline__ Real rcp_(Real v) { return Real(1) / v; }
+template <>
+__device__ __forceinline__ float rcp_<float>(float v) { return __builtin_amdgcn_rcpf(v); }
+
+__device__ __forceinline__ float exp_(float v) { return __expf(v); }
+__device__ __forceinline__ double exp_(double v) { return exp(v); }
+__device__ __forceinline__ float sqrt_(float v) { return __builtin_sqrtf(v); }
+__device__ __forceinline__ double sqrt_(double v) { return sqrt(v); }
+
+template <typename Real> struct TP;
+template <> struct TP<float> {
+  __device__ static float u0(const DTerm& t) { return t.fu0; }
+  __device__ static float r0(const DTerm& t) { return t.fr0; }
+  __device__ static float rc2(const DTerm& t) { return t.frc2; }
+  __device__ static float a_inv(const DTerm& t) { return t.fa_inv; }
+  __device__ static float rin(const DTerm& t) { return t.frin; }
+  __device__ static float c_lj(const DTerm& t) { return t.fc_lj; }
+  __device__ static float r0sq(const DTerm& t) { return t.fr0sq; }
+  __device__ static float slope(const DTerm& t) { return t.fslope; }
+};
+template <> struct TP<double> {
+  __device__ static double u0(const DTerm& t) { return t.u0; }
+  __device__ static double r0(const DTerm& t) { return t.r0; }
+  __device__ static double rc2(const DTerm& t) { return t.rc2; }
+  __device__ static double a_inv(const DTerm& t) { return t.a_inv; }
+  __device__ static double rin(const DTerm& t) { return t.rin; }
+  __device__ static double c_lj(const DTerm& t) { return t.c_lj; }
+  __device__ static double r0sq(const DTerm& t) { return t.r0sq; }
+  __device__ static double slope(const DTerm& t) { return t.slope; }
+};
+
+template <typename Real>
+__device__ __forceinline__ Real pair_coef(const DTerm& t, Real r2) {
+  typedef TP<Real> P;
+  switch (t.kind) {
+    case RDS_LJ: {
+      // f = -2n u0 / r * s (s - 1), s = (r0/r)^n  =>  c = -2n u0 s (s-1) / r^2
+      Real inv = rcp_(r2);
+      Real b = t.odd ? P::r0(t) * sqrt_(inv) : P::r0sq(t) * inv;
+      Real s = b;
+      for (int k = 1; k < t.pw; ++k) s *= b;
+      return P::c_lj(t) * inv * s * (s - Real(1));
+    }
+    case RDS_TANH: {
+      // f = u0 (1 - tanh^2((r - r0)/a)) / (2a); 1 - tanh^2(z) = 4e/(1+e)^2, e = exp(-2|z|)
+      Real r = sqrt_(r2);
+      Real z = (r - P::r0(t)) * P::a_inv(t);
+      Real e = exp_(Real(-2) * fabs(z));
+      Real q = Real(1) + e;
+      Real f = P::u0(t) * (Real(4) * e / (q * q)) * P::a_inv(t) / Real(2);
+      return f / r;
+    }
+    case RDS_YUKAWA: {
+      // f = -u0 (1 + r0/r) exp(-r/r0) / (r r0)
+      Real r = sqrt_(r2);
+      Real f = -P::u0(t) * (Real(1) + P::r0(t) / r) * exp_(-r / P::r0(t)) / (r * P::r0(t));
+      return f / r;
+    }
+    case RDS_HARMONIC:
+      return P::u0(t);  // f = u0 r
+    case RDS_PIECEWISE_LINEAR: {
+      Real r = sqrt_(r2);
+      return (r < P::rin(t)) ? Real(0) : P::slope(t) / r;
+    }
+  }
+  return Real(0);
+}
+
+// Wall force along its direction (pyrdsim.pyx:1134-1136, 1979-1986):
+// a[i, dir] += -u0 (1 - tanh^2((x - r0)/a)) / (2a)
+__device__ __forceinline__ double wall_force(const DWall& w, double x) {
+  double z = (x - w.r0) * w.a_inv;
+  double t = tanh(z);
+  return -(w.u0 * (1.0 - t * t) * w.a_inv / 2.0);
+}
+
+// ---------------------------------------------------------------------------
+// force (+ integrate + bin) kernel
+// ---------------------------------------------------------------------------
+
+enum { MODE_STEP = 0, MODE_FORCE = 1, MODE_COUNT = 2 };
+
+template <typename Real, bool MOL, int MODE>
+__global__ __launch_bounds__(FBLOCK) void k_force(const StepArgs A) {
+  __shared__ TileShared sh;
+  __shared__ double2 s_pos[CAP];
+  __shared__ int s_ty[CAP];
+  __shared__ int s_mol[MOL ? CAP : 1];
+  __shared__ int s_pfirst[MAXT * MAXT], s_pcount[MAXT * MAXT];
+  __shared__ DTerm s_term[MAXTERMS];
+
+  const int lane = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, A.n_tiles);
+  const int4 T = A.tiles[tile];
+  const DRep& R = A.rep[T.x];
+  const DModel& M = A.models[R.model];
+  const int cy = T.y, cx0 = T.z, W = T.w;
+  const int nt = M.nt;
+
+  for (int q = lane; q < nt * nt; q += FBLOCK) {
+    s_pfirst[q] = M.pfirst[q];
+    s_pcount[q] = M.pcount[q];
+  }
+  for (int q = lane; q < M.nterms; q += FBLOCK) s_term[q] = M.terms[q];
+  tile_layout(R, A.cstart, cy, cx0, W, sh);
+
+  const int total = sh.total;
+  const bool resident = total <= CAP;
+  if (resident) {
+    tile_stage<MOL>(sh, 0, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+  }
+  __syncthreads();
+
+  long long step_next = 0;
+  if (MODE == MODE_STEP) step_next = *A.step + 1;
+
+  for (int hb = sh.home_begin; hb < sh.home_end; hb += FBLOCK) {
+    const int s = hb + lane;
+    const bool valid = s < sh.home_end;
+    double2 xi = make_double2(0.0, 0.0);
+    int ti = 0, mi = 0, w = 0, own = -1;
+    if (valid) {
+      xi = A.pos[s];
+      ti = A.type[s];
+      if (MOL) mi = A.mol[s];
+      int cx = cell_coord(xi.x, R.cl[0], R.lx);
+      w = cx - cx0;
+      w = min(max(w, 0), W - 1);
+      own = sh.mid_off1 + (s - sh.mid_slot1);
+    }
+    int wlo[3], whi[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      wlo[r] = valid ? sh.row_base[r] + sh.off[r][w] : 0;
+      whi[r] = valid ? sh.row_base[r] + sh.off[r][w + 3] : 0;
+    }
+    Real fx = 0, fy = 0;
+    int ncount = 0;
+    const int tbase = ti * nt;
+
+    for (int p0 = 0; p0 < total; p0 += CAP) {
+      if (!resident) {
+        __syncthreads();
+        tile_stage<MOL>(sh, p0, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+        __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        int lo = max(wlo[r], p0) - p0;
+        int hi = min(whi[r], p0 + CAP) - p0;
+        for (int j = lo; j < hi; ++j) {
+          double2 pj = s_pos[j];
+          double dxd = pj.x - xi.x, dyd = pj.y - xi.y;
+          bool other = MOL ? (s_mol[j] != mi) : (j + p0 != own);
+          if (MODE == MODE_COUNT) {
+            double d2 = dxd * dxd + dyd * dyd;
+            ncount += (other && d2 <= A.rprobe2) ? 1 : 0;
+          } else {
+            Real dx = (Real)dxd, dy = (Real)dyd;
+            Real r2 = dx * dx + dy * dy;
+            int pq = tbase + s_ty[j];
+            int q0 = s_pfirst[pq], qn = s_pcount[pq];
+            for (int q = q0; q < q0 + qn; ++q) {
+              const DTerm& t = s_term[q];
+              Real c = pair_coef<Real>(t, r2);
+              bool in = other && (r2 <= TP<Real>::rc2(t));
+              c = in ? c : Real(0);
+              fx += c * dx;
+              fy += c * dy;
+            }
+          }
+        }
+      }
+    }
+
+    if (!valid) continue;
+    if (MODE == MODE_COUNT) {
+      A.iout[A.pid[s]] = ncount;
+      continue;
+    }
+    double Fx = (double)fx, Fy = (double)fy;
+    for (int q = 0; q < M.nwalls; ++q) {
+      const DWall& wl = M.walls[q];
+      if (wl.type != ti) continue;
+      if (wl.dir == 0) Fx += wall_force(wl, xi.x);
+      else Fy += wall_force(wl, xi.y);
+    }
+    if (A.fbond) {
+      double2 fb = A.fbond[s];
+      Fx += fb.x;
+      Fy += fb.y;
+    }
+    const int p = A.pid[s];
+    if (MODE == MODE_FORCE) {
+      A.fout[p] = make_double2(Fx, Fy);
+      continue;
+    }
+
+    // Euler-Maruyama, pyrdsim.pyx:2244-2250: two separate FP64 additions.
+    double n0, n1;
+    if (A.rng == RDS_RNG_INJECTED) {
+      long long o = step_next - A.inj_first;
+      if (o >= 0 && o < A.inj_count) {
+        const double* src = A.inj_n + (o * A.n_total + p) * 2;
+        n0 = src[0];
+        n1 = src[1];
+      } else {
+        n0 = n1 = 0.0;
+        A.status->stream = 1;
+      }
+    } else {
+      float a0, a1;
+      philox_normals(R, p - R.pbase, step_next, a0, a1);
+      n0 = a0;
+      n1 = a1;
+    }
+    const double fp = R.fpre[ti];
+    const double np = A.include_rf * R.npre[ti];
+    double x = xi.x, y = xi.y;
+    {
+#pragma clang fp contract(off)
+      x = x + fp * Fx;
+      x = x + np * n0;
+      y = y + fp * Fy;
+      y = y + np * n1;
+    }
+    // boundary conditions, pyrdsim.pyx:2269-2292
+    double Ls[2] = {R.L[0], R.L[1]};
+    int per[2] = {R.px, R.py};
+    double v[2] = {x, y};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      double L = Ls[k];
+      double u = v[k];
+      if (!per[k]) {
+        if (u > L) u -= 2.0 * (u - L);
+        if (u < 0.0) u -= 2.0 * u;
+      } else {
+        // Python-style modulo as Cython emits it for doubles
+        double m = fmod(u, L);
+        if (m != 0.0 && ((m < 0.0) != (L < 0.0))) m += L;
+        u = m;
+      }
+      if (u < 0.0 || u > L || u != u) {
+        unsigned long long key = ((unsigned long long)step_next << 32) |
+                                 (unsigned long long)(unsigned)(p * 2 + k);
+        unsigned long long old = atomicMin(&A.status->err_key, key);
+        if (key < old) A.status->err_val = u;
+        A.status->bounds = 1;
+      }
+      v[k] = u;
+    }
+    int cx = cell_coord(v[0], R.cl[0], R.lx);
+    int cyy = cell_coord(v[1], R.cl[1], R.ly);
+    int c = R.cbase + cyy * R.lx + cx;
+    A.upos[s] = make_double2(v[0], v[1]);
+    A.ucell[s] = c;
+    A.urank[s] = atomicAdd(&A.count[c], 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// reaction kernel (pyrdsim.pyx:2070-2222), exact reference arithmetic: rates are
+// accumulated by repeated addition in reaction order, as the reference does.
+// ---------------------------------------------------------------------------
+
+template <bool MOL>
+__global__ __launch_bounds__(FBLOCK) void k_react(const StepArgs A) {
+  __shared__ TileShared sh;
+  __shared__ double2 s_pos[CAP];
+  __shared__ int s_ty[CAP];
+  __shared__ int s_mol[MOL ? CAP : 1];
+
+  const int lane = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, A.n_tiles);
+  const int4 T = A.tiles[tile];
+  const DRep& R = A.rep[T.x];
+  const DModel& M = A.models[R.model];
+  const int cy = T.y, cx0 = T.z, W = T.w;
+  const int nrx = M.nrx;
+  const long long step = *A.step;  // already advanced: the reference's l+1
+
+  tile_layout(R, A.cstart, cy, cx0, W, sh);
+  const int total = sh.total;
+  const bool need_nbr = M.nbr_rx != 0;
+  const bool resident = total <= CAP;
+  if (need_nbr && resident) tile_stage<MOL>(sh, 0, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+  __syncthreads();
+
+  for (int hb = sh.home_begin; hb < sh.home_end; hb += FBLOCK) {
+    const int s = hb + lane;
+    const bool valid = s < sh.home_end;
+    double2 xi = make_double2(0.0, 0.0);
+    int ti = -1, mi = 0, w = 0, own = -1;
+    bool counts_needed = false;
+    if (valid) {
+      xi = A.pos[s];
+      ti = A.type[s];
+      if (MOL) mi = A.mol[s];
+      int cx = cell_coord(xi.x, R.cl[0], R.lx);
+      w = min(max(cx - cx0, 0), W - 1);
+      own = sh.mid_off1 + (s - sh.mid_slot1);
+      for (int q = 0; q < nrx; ++q)
+        if (M.rx[q].kind != RDS_RX_LINEAR && M.rx[q].reactant == ti) counts_needed = true;
+    }
+    int cnt[MAXRX];
+#pragma unroll
+    for (int q = 0; q < MAXRX; ++q) cnt[q] = 0;
+    if (need_nbr) {
+      int wlo[3], whi[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        wlo[r] = counts_needed ? sh.row_base[r] + sh.off[r][w] : 0;
+        whi[r] = counts_needed ? sh.row_base[r] + sh.off[r][w + 3] : 0;
+      }
+      for (int p0 = 0; p0 < total; p0 += CAP) {
+        if (!resident) {
+          __syncthreads();
+          tile_stage<MOL>(sh, p0, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+          __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          int lo = max(wlo[r], p0) - p0;
+          int hi = min(whi[r], p0 + CAP) - p0;
+          for (int j = lo; j < hi; ++j) {
+            bool other = MOL ? (s_mol[j] != mi) : (j + p0 != own);
+            if (!other) continue;
+            double2 pj = s_pos[j];
+            double dx = pj.x - xi.x, dy = pj.y - xi.y;
+            double d = sqrt(dx * dx + dy * dy);
+            int tj = s_ty[j];
+#pragma unroll
+            for (int q = 0; q < MAXRX; ++q) {
+              if (q >= nrx) break;
+              const DRx& rx = M.rx[q];
+              if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
+              bool match = (rx.kind == RDS_RX_CATALYZED) ? (tj == rx.catalyst)
+                                                         : (tj == rx.inh || tj == rx.inh2);
+              if (match && d < rx.r0) cnt[q] += 1;
+            }
+          }
+        }
+      }
+    }
+    if (!valid) continue;
+    // rates in reaction order (pyrdsim.pyx:2103-2206)
+    double rates[MAXRX];
+    double total_rate = 0.0;
+#pragma unroll
+    for (int q = 0; q < MAXRX; ++q) rates[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < MAXRX; ++q) {
+      if (q >= nrx) break;
+      const DRx& rx = M.rx[q];
+      const double rate = R.rate[q];
+      if (rx.reactant != ti) continue;
+      if (rx.kind == RDS_RX_LINEAR) {
+        rates[q] = rate;
+        total_rate += rate;
+      } else if (rx.kind == RDS_RX_CATALYZED) {
+        double acc = 0.0;
+        for (int k = 0; k < cnt[q]; ++k) {
+          acc += rate;
+          total_rate += rate;
+        }
+        rates[q] = acc;
+      } else {
+        double acc = rate;
+        for (int k = 0; k < cnt[q]; ++k) acc -= rx.rate_decrease;
+        if (acc < 0) acc = 0.0;
+        rates[q] = acc;
+        total_rate += acc;
+      }
+    }
+    int tnew = ti;
+    if (total_rate > 0.0) {
+      const int p = A.pid[s];
+      double u1, u2;
+      if (A.rng == RDS_RNG_INJECTED) {
+        long long o = step - A.inj_first;
+        if (o >= 0 && o < A.inj_count) {
+          u1 = A.inj_u1[o * A.n_total + p];
+          u2 = A.inj_u2 ? A.inj_u2[o * A.n_total + p] : 0.0;
+        } else {
+          u1 = 2.0;
+          u2 = 0.0;
+          A.status->stream = 1;
+        }
+      } else {
+        philox_uniforms(R, p - R.pbase, step, u1, u2);
+      }
+      double prob = (A.prob_mode == RDS_PROB_EXP) ? -expm1(-R.dt * total_rate)
+                                                  : R.dt * total_rate;
+      if (prob > u1) {
+        double sacc = 0.0;
+        int chosen = -1;
+#pragma unroll
+        for (int q = 0; q < MAXRX; ++q) {
+          if (q >= nrx || chosen >= 0) break;
+          sacc += rates[q] / total_rate;
+          if (sacc >= u2) chosen = q;
+        }
+        if (chosen >= 0) {
+          tnew = M.rx[chosen].product;
+          atomicAdd(&A.status->events, 1ull);
+        }
+      }
+    }
+    A.type_out[s] = tnew;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// harmonic bonds (pyrdsim.pyx:1303-1320, 2037-2047), FP64, per slot
+// ---------------------------------------------------------------------------
+
+__global__ void k_bonds(int n_total, const DRep* __restrict__ rep, const int* __restrict__ rep_of,
+                        const double2* __restrict__ pos, const int* __restrict__ pid,
+                        const int* __restrict__ slot_of, const int* __restrict__ boff,
+                        const int* __restrict__ bpart, const double* __restrict__ br0,
+                        const double* __restrict__ bu0, double2* __restrict__ fbond) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  int p = pid[s];
+  const DRep& R = rep[rep_of[p]];
+  double2 xi = pos[s];
+  double fx = 0.0, fy = 0.0;
+  for (int b = boff[p]; b < boff[p + 1]; ++b) {
+    double2 xj = pos[slot_of[bpart[b]]];
+    double dx = xj.x - xi.x, dy = xj.y - xi.y;
+    if (R.px) dx = dx - R.L[0] * round(dx / R.L[0]);
+    if (R.py) dy = dy - R.L[1] * round(dy / R.L[1]);
+    double d = sqrt(dx * dx + dy * dy);
+    double f = bu0[b] * (d - br0[b]);
+    fx += f * (dx / d);
+    fy += f * (dy / d);
+  }
+  fbond[s] = make_double2(fx, fy);
+}
+
+// ---------------------------------------------------------------------------
+// binning, scan, scatter, output
+// ---------------------------------------------------------------------------
+
+__global__ void k_bin(int n_total, const DRep* __restrict__ rep, const int* __restrict__ rep_of,
+                      const double2* __restrict__ pos, int* __restrict__ ucell,
+                      int* __restrict__ urank, int* __restrict__ count) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  const DRep& R = rep[rep_of[s]];
+  double2 x = pos[s];
+  int c = R.cbase + cell_coord(x.y, R.cl[1], R.ly) * R.lx + cell_coord(x.x, R.cl[0], R.lx);
+  ucell[s] = c;
+  urank[s] = atomicAdd(&count[c], 1);
+}
+
+#define SCAN_BLOCK 256
+#define SCAN_ITEMS 8
+
+// One workgroup per replica: cell starts = pbase + exclusive prefix of the
+// histogram; the histogram is cleared for the next step.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const DRep* __restrict__ rep,
+                                                     int* __restrict__ count,
+                                                     int* __restrict__ cstart,
+                                                     long long* step, int advance) {
+  __shared__ int s_wsum[SCAN_BLOCK / 64];
+  __shared__ int s_total;
+  const DRep& R = rep[blockIdx.x];
+  const int t = threadIdx.x;
+  const int C = R.lx * R.ly;
+  const int cb = R.cbase;
+  int running = R.pbase;
+  for (int base = 0; base < C; base += SCAN_BLOCK * SCAN_ITEMS) {
+    int v[SCAN_ITEMS];
+    int tsum = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+      int c = base + t * SCAN_ITEMS + i;
+      v[i] = (c < C) ? count[cb + c] : 0;
+      tsum += v[i];
+    }
+    // wave inclusive scan of tsum
+    int inc = tsum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      int o = __shfl_up(inc, d, 64);
+      if ((t & 63) >= d) inc += o;
+    }
+    if ((t & 63) == 63) s_wsum[t >> 6] = inc;
+    __syncthreads();
+    if (t == 0) {
+      int acc = 0;
+      for (int k = 0; k < SCAN_BLOCK / 64; ++k) {
+        int x = s_wsum[k];
+        s_wsum[k] = acc;
+        acc += x;
+      }
+      s_total = acc;
+    }
+    __syncthreads();
+    int excl = running + s_wsum[t >> 6] + inc - tsum;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+      int c = base + t * SCAN_ITEMS + i;
+      if (c < C) {
+        cstart[cb + c] = excl;
+        count[cb + c] = 0;
+      }
+      excl += v[i];
+    }
+    running += s_total;
+    __syncthreads();
+  }
+  if (advance && blockIdx.x == 0 && t == 0) *step = *step + 1;
+}
+
+__global__ void k_scatter(int n_total, const double2* __restrict__ upos,
+                          const int* __restrict__ ucell, const int* __restrict__ urank,
+                          const int* __restrict__ cstart, const int* __restrict__ type,
+                          const int* __restrict__ pid, const int* __restrict__ mol,
+                          double2* __restrict__ npos, int* __restrict__ ntype,
+                          int* __restrict__ npid, int* __restrict__ nmol,
+                          int* __restrict__ slot_of) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  int d = cstart[ucell[s]] + urank[s];
+  int p = pid[s];
+  npos[d] = upos[s];
+  ntype[d] = type[s];
+  npid[d] = p;
+  if (nmol) nmol[d] = mol[s];
+  if (slot_of) slot_of[p] = d;
+}
+
+__global__ void k_gather_out(int n_total, const double2* __restrict__ pos,
+                             const int* __restrict__ type, const int* __restrict__ pid,
+                             double2* __restrict__ xo, int* __restrict__ to) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  int p = pid[s];
+  if (xo) xo[p] = pos[s];
+  if (to) to[p] = type[s];
+}
+
+__global__ void k_cells_out(int n_total, const DRep* __restrict__ rep,
+                            const int* __restrict__ rep_of, const double2* __restrict__ pos,
+                            const int* __restrict__ pid, int* __restrict__ out) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  int p = pid[s];
+  const DRep& R = rep[rep_of[p]];
+  double2 x = pos[s];
+  out[p] = R.cbase + cell_coord(x.y, R.cl[1], R.ly) * R.lx + cell_coord(x.x, R.cl[0], R.lx);
+}
+
+__global__ void k_load(int n_total, const double2* __restrict__ x, const int* __restrict__ t,
+                       const int* __restrict__ m, double2* __restrict__ pos,
+                       int* __restrict__ type, int* __restrict__ pid, int* __restrict__ mol) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  pos[s] = x[s];
+  type[s] = t[s];
+  pid[s] = s;
+  if (mol) mol[s] = m[s];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct Buf {
+  double2* pos = nullptr;
+  int* type = nullptr;
+  int* type0 = nullptr;  // pre-reaction types written by the scatter
+  int* pid = nullptr;
+  int* mol = nullptr;
+};
+
+struct rds_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  int precision = RDS_MIXED, rng = RDS_RNG_PHILOX, prob_mode = RDS_PROB_LINEAR;
+  int has_mol = 0;
+  int graph_steps = 0;
+  double include_rf = 1.0;
+  int n_rep = 0, n_models = 0;
+  int64_t n_total = 0, n_cells = 0, n_tiles = 0;
+  bool any_rx = false, any_bonds = false;
+  std::vector<DRep> h_rep;
+  std::vector<DModel> h_models;
+  std::vector<int4> h_tiles;
+  DRep* d_rep = nullptr;
+  DModel* d_models = nullptr;
+  int4* d_tiles = nullptr;
+  int* d_rep_of = nullptr;  // replica of each pid
+  Buf buf[2];
+  int parity = 0;
+  double2* upos = nullptr;
+  int* ucell = nullptr;
+  int* urank = nullptr;
+  int* count = nullptr;
+  int* cstart = nullptr;
+  long long* d_step = nullptr;
+  DStatus* d_status = nullptr;
+  // bonds
+  int* boff = nullptr;
+  int* bpart = nullptr;
+  double* br0 = nullptr;
+  double* bu0 = nullptr;
+  double2* fbond = nullptr;
+  int* slot_of = nullptr;
+  // injected streams
+  const double* inj_n = nullptr;
+  const double* inj_u1 = nullptr;
+  const double* inj_u2 = nullptr;
+  long long inj_first = 0, inj_count = 0;
+  // state
+  bool loaded = false;
+  long long step0 = 0;
+  long long steps_done = 0;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_len = 0;
+  int graph_parity = 0;
+};
+
+namespace {
+
+StepArgs make_args(rds_ctx* c, const Buf& cur) {
+  StepArgs a;
+  memset(&a, 0, sizeof a);
+  a.rep = c->d_rep;
+  a.models = c->d_models;
+  a.tiles = c->d_tiles;
+  a.n_tiles = (int)c->n_tiles;
+  a.n_total = (int)c->n_total;
+  a.cstart = c->cstart;
+  a.pos = cur.pos;
+  a.type = cur.type;
+  a.pid = cur.pid;
+  a.mol = cur.mol;
+  a.upos = c->upos;
+  a.ucell = c->ucell;
+  a.urank = c->urank;
+  a.count = c->count;
+  a.fbond = c->any_bonds ? c->fbond : nullptr;
+  a.step = c->d_step;
+  a.inj_n = c->inj_n;
+  a.inj_u1 = c->inj_u1;
+  a.inj_u2 = c->inj_u2;
+  a.inj_first = c->inj_first;
+  a.inj_count = c->inj_count;
+  a.rng = c->rng;
+  a.prob_mode = c->prob_mode;
+  a.include_rf = c->include_rf;
+  a.status = c->d_status;
+  return a;
+}
+
+template <int MODE>
+void launch_force(rds_ctx* c, const StepArgs& a) {
+  dim3 g((unsigned)c->n_tiles), b(FBLOCK);
+  if (c->precision == RDS_FP64) {
+    if (c->has_mol) hipLaunchKernelGGL((k_force<double, true, MODE>), g, b, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_force<double, false, MODE>), g, b, 0, c->stream, a);
+  } else {
+    if (c->has_mol) hipLaunchKernelGGL((k_force<float, true, MODE>), g, b, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_force<float, false, MODE>), g, b, 0, c->stream, a);
+  }
+}
+
+inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// kernel ids for timing
+enum { KT_FORCE = 0, KT_SCAN = 1, KT_SCATTER = 2, KT_REACT = 3, KT_BONDS = 4, KT_N = 5 };
+
+struct Timer {
+  hipEvent_t ev[2 * KT_N * 2];
+  bool on = false;
+};
+
+// One integration step on the context's stream.  `ev` (optional) receives event
+// pairs around each launch.
+void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>* ev) {
+  Buf& cur = c->buf[c->parity];
+  Buf& nxt = c->buf[c->parity ^ 1];
+  auto mark = [&](int kid, bool begin) {
+    if (!ev) return;
+    if (begin) {
+      hipEvent_t e0, e1;
+      hipEventCreate(&e0);
+      hipEventCreate(&e1);
+      ev->push_back({kid, {e0, e1}});
+      hipEventRecord(e0, c->stream);
+    } else {
+      hipEventRecord(ev->back().second.second, c->stream);
+    }
+  };
+  if (c->any_bonds) {
+    mark(KT_BONDS, true);
+    hipLaunchKernelGGL(k_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                       (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, c->slot_of,
+                       c->boff, c->bpart, c->br0, c->bu0, c->fbond);
+    mark(KT_BONDS, false);
+  }
+  StepArgs a = make_args(c, cur);
+  mark(KT_FORCE, true);
+  launch_force<MODE_STEP>(c, a);
+  mark(KT_FORCE, false);
+  mark(KT_SCAN, true);
+  hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
+                     c->cstart, c->d_step, 1);
+  mark(KT_SCAN, false);
+  mark(KT_SCATTER, true);
+  hipLaunchKernelGGL(k_scatter, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                     (int)c->n_total, c->upos, c->ucell, c->urank, c->cstart, cur.type, cur.pid,
+                     cur.mol, nxt.pos, c->any_rx ? nxt.type0 : nxt.type, nxt.pid, nxt.mol,
+                     c->any_bonds ? c->slot_of : nullptr);
+  mark(KT_SCATTER, false);
+  if (c->any_rx) {
+    StepArgs r = make_args(c, nxt);
+    r.type = nxt.type0;
+    r.type_out = nxt.type;
+    mark(KT_REACT, true);
+    if (c->has_mol) hipLaunchKernelGGL((k_react<true>), dim3((unsigned)c->n_tiles), dim3(FBLOCK), 0, c->stream, r);
+    else hipLaunchKernelGGL((k_react<false>), dim3((unsigned)c->n_tiles), dim3(FBLOCK), 0, c->stream, r);
+    mark(KT_REACT, false);
+  }
+  c->parity ^= 1;
+}
+
+void free_graph(rds_ctx* c) {
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_len = 0;
+}
+
+int build_graph(rds_ctx* c) {
+  free_graph(c);
+  int g = c->graph_steps;
+  if (g <= 0) return RDS_OK;
+  if (g % 2) ++g;
+  const int par = c->parity;
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < g; ++k) enqueue_step(c, nullptr);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  c->parity = par;  // capture did not execute anything
+  if (e != hipSuccess) return fail(RDS_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  c->graph = graph;
+  HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+  c->graph_len = g;
+  c->graph_parity = par;
+  return RDS_OK;
+}
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+  if (e != hipSuccess)
+    return fail(RDS_E_HIP, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+  return RDS_OK;
+}
+
+#define RDS_TRY(expr)          \
+  do {                         \
+    int rc_ = (expr);          \
+    if (rc_ != RDS_OK) return rc_; \
+  } while (0)
+
+int fill_model(const rds_model& m, DModel& d, double* rcut) {
+  memset(&d, 0, sizeof d);
+  if (m.n_types < 1 || m.n_types > MAXT) return fail(RDS_E_ARG, "n_types %d outside [1, %d]", m.n_types, MAXT);
+  if (m.n_terms < 0 || m.n_terms > MAXTERMS) return fail(RDS_E_ARG, "n_terms %d outside [0, %d]", m.n_terms, MAXTERMS);
+  if (m.n_walls < 0 || m.n_walls > MAXWALLS) return fail(RDS_E_ARG, "n_walls %d outside [0, %d]", m.n_walls, MAXWALLS);
+  if (m.n_reactions < 0 || m.n_reactions > MAXRX) return fail(RDS_E_ARG, "n_reactions %d outside [0, %d]", m.n_reactions, MAXRX);
+  const int nt = m.n_types;
+  d.nt = nt;
+  d.nwalls = m.n_walls;
+  d.nrx = m.n_reactions;
+  double rc = 0.0;
+  // terms grouped per unordered type pair, file order kept inside a group
+  int k = 0;
+  for (int a = 0; a < nt; ++a)
+    for (int b = a; b < nt; ++b) {
+      int first = k;
+      for (int q = 0; q < m.n_terms; ++q) {
+        const rds_pair_term& t = m.terms[q];
+        if (t.type_a != a || t.type_b != b) continue;
+        DTerm& o = d.terms[k++];
+        memset(&o, 0, sizeof o);
+        o.kind = t.kind;
+        o.n = t.n;
+        double rcv = t.rc;
+        if (t.kind == RDS_PIECEWISE_LINEAR) rcv = t.r0;  // pyrdsim.pyx:1057-1060
+        if (t.kind == RDS_LJ) {
+          if (t.n < 1) return fail(RDS_E_ARG, "lj exponent n must be >= 1");
+          o.odd = t.n % 2;
+          o.pw = o.odd ? t.n : t.n / 2;
+          o.c_lj = -2.0 * t.n * t.u0;
+        } else if (t.kind == RDS_TANH) {
+          o.a_inv = 1.0 / t.a;
+        } else if (t.kind == RDS_PIECEWISE_LINEAR) {
+          o.rin = t.r0 - t.dr;
+          o.slope = t.u0 / t.dr;
+        } else if (t.kind != RDS_YUKAWA && t.kind != RDS_HARMONIC) {
+          return fail(RDS_E_ARG, "unknown pair kind %d", t.kind);
+        }
+        o.u0 = t.u0;
+        o.r0 = t.r0;
+        o.r0sq = t.r0 * t.r0;
+        o.rc2 = rcv * rcv;
+        o.fu0 = (float)o.u0;
+        o.fr0 = (float)o.r0;
+        o.frc2 = (float)rcv * (float)rcv;
+        o.fa_inv = (float)o.a_inv;
+        o.frin = (float)o.rin;
+        o.fc_lj = (float)o.c_lj;
+        o.fr0sq = (float)o.r0 * (float)o.r0;
+        o.fslope = (float)o.slope;
+        rc = std::max(rc, rcv);
+      }
+      d.pfirst[a * nt + b] = d.pfirst[b * nt + a] = first;
+      d.pcount[a * nt + b] = d.pcount[b * nt + a] = k - first;
+    }
+  // terms whose pair is not sorted / out of range never match (pyrdsim.pyx:1999-2007)
+  d.nterms = k;
+  for (int q = 0; q < m.n_walls; ++q) {
+    const rds_wall_term& w = m.walls[q];
+    if (w.direction < 0 || w.direction > 1) return fail(RDS_E_ARG, "wall direction must be 0 or 1");
+    d.walls[q].type = w.particle_type;
+    d.walls[q].dir = w.direction;
+    d.walls[q].u0 = w.u0;
+    d.walls[q].r0 = w.r0;
+    d.walls[q].a_inv = 1.0 / w.a;
+  }
+  for (int q = 0; q < m.n_reactions; ++q) {
+    const rds_reaction_def& r = m.reactions[q];
+    DRx& o = d.rx[q];
+    o.kind = r.kind;
+    o.reactant = r.reactant;
+    o.product = r.product;
+    o.catalyst = r.catalyst;
+    o.inh = r.inhibitor;
+    o.inh2 = r.inhibitor2;
+    o.r0 = r.r0;
+    o.rate_decrease = r.rate_decrease;
+    if (r.reactant < 0 || r.reactant >= nt || r.product < 0 || r.product >= nt)
+      return fail(RDS_E_ARG, "reaction %d: reactant/product outside the %d particle types", q, nt);
+    if (r.kind == RDS_RX_CATALYZED || r.kind == RDS_RX_INHIBITED) {
+      d.nbr_rx = 1;
+      rc = std::max(rc, r.r0);
+    } else if (r.kind != RDS_RX_LINEAR) {
+      return fail(RDS_E_ARG, "unknown reaction kind %d", r.kind);
+    }
+  }
+  *rcut = rc;
+  return RDS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rds_abi_version(void) { return RDS_ABI_VERSION; }
+
+const char* rds_last_error(void) { return g_err.c_str(); }
+
+int rds_ctx_destroy(rds_ctx* c) {
+  if (!c) return RDS_OK;
+  free_graph(c);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->upos, c->ucell, c->urank,
+                  c->count, c->cstart, c->d_step, c->d_status, c->boff, c->bpart, c->br0,
+                  c->bu0, c->fbond, c->slot_of};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  for (int k = 0; k < 2; ++k) {
+    void* b[] = {c->buf[k].pos, c->buf[k].type, c->buf[k].type0, c->buf[k].pid, c->buf[k].mol};
+    for (void* p : b)
+      if (p) hipFree(p);
+  }
+  if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
+  delete c;
+  return RDS_OK;
+}
+
+int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
+  if (!cfg || !out) return fail(RDS_E_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->n_models < 1 || cfg->n_replicas < 1 || !cfg->models || !cfg->replicas)
+    return fail(RDS_E_ARG, "need at least one model and one replica");
+  if (cfg->precision != RDS_FP64 && cfg->precision != RDS_MIXED)
+    return fail(RDS_E_ARG, "unknown precision %d", cfg->precision);
+  if (cfg->rng != RDS_RNG_PHILOX && cfg->rng != RDS_RNG_INJECTED)
+    return fail(RDS_E_ARG, "unknown rng %d", cfg->rng);
+  rds_ctx* c = new rds_ctx();
+  int rc = RDS_OK;
+  auto bail = [&](int code) {
+    rds_ctx_destroy(c);
+    return code;
+  };
+  c->precision = cfg->precision;
+  c->rng = cfg->rng;
+  c->prob_mode = cfg->reaction_prob;
+  c->has_mol = cfg->has_molecules ? 1 : 0;
+  c->graph_steps = cfg->graph_steps;
+  c->n_rep = cfg->n_replicas;
+  c->n_models = cfg->n_models;
+  const int spc = cfg->cells_per_cutoff > 0 ? cfg->cells_per_cutoff : 1;
+  if (spc != 1) return bail(fail(RDS_E_ARG, "only cells_per_cutoff = 1 (3x3 stencil) is built"));
+  const double target = cfg->tile_particles > 0 ? cfg->tile_particles : 56;
+
+  std::vector<double> rcut(cfg->n_models);
+  c->h_models.resize(cfg->n_models);
+  for (int m = 0; m < cfg->n_models; ++m) {
+    rc = fill_model(cfg->models[m], c->h_models[m], &rcut[m]);
+    if (rc) return bail(rc);
+    if (cfg->models[m].n_reactions > 0) c->any_rx = true;
+  }
+  if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(RDS_E_HIP, "no HIP device"));
+  c->h_rep.resize(c->n_rep);
+  int64_t pbase = 0, cbase = 0;
+  for (int r = 0; r < c->n_rep; ++r) {
+    const rds_replica& s = cfg->replicas[r];
+    DRep& d = c->h_rep[r];
+    memset(&d, 0, sizeof d);
+    if (s.model < 0 || s.model >= cfg->n_models) return bail(fail(RDS_E_ARG, "replica %d: bad model", r));
+    if (s.n_particles < 1) return bail(fail(RDS_E_ARG, "replica %d: no particles", r));
+    double rcm = rcut[s.model];
+    if (!(rcm > 0.0)) rcm = std::min(s.L[0], s.L[1]);  // no pair range: one cell suffices
+    int lcell[2];
+    for (int k = 0; k < 2; ++k) {
+      if (!(s.L[k] > 0.0)) return bail(fail(RDS_E_ARG, "replica %d: box length must be > 0", r));
+      // largest l with L/l >= rcut (the reference's rule, pyrdsim.pyx:1439-1441,
+      // with rs = the largest interaction range and ratio = 1)
+      int l = 1;
+      while (s.L[k] / (double)(l + 1) >= rcm) ++l;
+      if (s.periodic[k] && l < 3) {
+        if (rcut[s.model] > 0.0)
+          return bail(fail(RDS_E_ARG,
+                           "replica %d: periodic box length %g holds fewer than 3 cells of the "
+                           "interaction range %g",
+                           r, s.L[k], rcut[s.model]));
+      }
+      lcell[k] = l;
+      d.L[k] = s.L[k];
+      d.cl[k] = s.L[k] / (double)l;
+    }
+    // a periodic axis with fewer than 3 cells (only possible without any pair
+    // range) is treated as one cell with no neighbours: nothing interacts
+    d.lx = lcell[0];
+    d.ly = lcell[1];
+    d.px = s.periodic[0] ? 1 : 0;
+    d.py = s.periodic[1] ? 1 : 0;
+    d.sx = (d.px && d.lx >= 3) ? 1 : 0;
+    d.sy = (d.py && d.ly >= 3) ? 1 : 0;
+    d.cbase = (int)cbase;
+    d.pbase = (int)pbase;
+    d.n = s.n_particles;
+    d.model = s.model;
+    d.seed_lo = (unsigned)(s.seed & 0xffffffffu);
+    d.seed_hi = (unsigned)(s.seed >> 32);
+    for (int t = 0; t < MAXT; ++t) {
+      d.fpre[t] = s.force_prefac[t];
+      d.npre[t] = s.noise_prefac[t];
+    }
+    for (int q = 0; q < MAXRX; ++q) d.rate[q] = s.rate[q];
+    d.dt = s.dt;
+    // tiles: strips of W cells per row
+    double occ = (double)s.n_particles / ((double)d.lx * d.ly);
+    int W = (int)std::floor(target / std::max(occ, 1e-9));
+    W = std::max(1, std::min(W, std::min(MAXW, d.lx)));
+    int nstrip = (d.lx + W - 1) / W;
+    int Wq = d.lx / nstrip, Wr = d.lx % nstrip;
+    for (int y = 0; y < d.ly; ++y) {
+      int x0 = 0;
+      for (int k = 0; k < nstrip; ++k) {
+        int wk = Wq + (k < Wr ? 1 : 0);
+        c->h_tiles.push_back(make_int4(r, y, x0, wk));
+        x0 += wk;
+      }
+    }
+    pbase += s.n_particles;
+    cbase += (int64_t)d.lx * d.ly;
+    if (pbase > 0x7fffffff || cbase > 0x7ffffff0)
+      return bail(fail(RDS_E_ARG, "too many particles or cells for 32-bit indices"));
+  }
+  c->n_total = pbase;
+  c->n_cells = cbase;
+  c->n_tiles = (int64_t)c->h_tiles.size();
+
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const size_t N = (size_t)c->n_total, C = (size_t)c->n_cells;
+#define A_(p, n) \
+  if ((rc = dalloc(&(p), (n))) != RDS_OK) return bail(rc)
+  A_(c->d_rep, c->n_rep);
+  A_(c->d_models, c->n_models);
+  A_(c->d_tiles, (size_t)c->n_tiles);
+  A_(c->d_rep_of, N);
+  for (int k = 0; k < 2; ++k) {
+    A_(c->buf[k].pos, N);
+    A_(c->buf[k].type, N);
+    A_(c->buf[k].type0, N);
+    A_(c->buf[k].pid, N);
+    if (c->has_mol) A_(c->buf[k].mol, N);
+  }
+  A_(c->upos, N);
+  A_(c->ucell, N);
+  A_(c->urank, N);
+  A_(c->count, C);
+  A_(c->cstart, C + 1);
+  A_(c->d_step, 1);
+  A_(c->d_status, 1);
+  A_(c->slot_of, N);
+
+  HIP_TRY(hipMemcpy(c->d_rep, c->h_rep.data(), sizeof(DRep) * c->n_rep, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_models, c->h_models.data(), sizeof(DModel) * c->n_models, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_tiles, c->h_tiles.data(), sizeof(int4) * c->n_tiles, hipMemcpyHostToDevice));
+  {
+    std::vector<int> rep_of(N);
+    for (int r = 0; r < c->n_rep; ++r)
+      for (int i = 0; i < c->h_rep[r].n; ++i) rep_of[c->h_rep[r].pbase + i] = r;
+    HIP_TRY(hipMemcpy(c->d_rep_of, rep_of.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipMemset(c->count, 0, sizeof(int) * C));
+  {
+    int nt = (int)N;
+    HIP_TRY(hipMemcpy(c->cstart + C, &nt, sizeof(int), hipMemcpyHostToDevice));
+  }
+
+  // bonds: CSR by particle, both directions
+  if (cfg->n_bonds > 0) {
+    if (!cfg->bond_i || !cfg->bond_j || !cfg->bond_r0 || !cfg->bond_u0)
+      return bail(fail(RDS_E_ARG, "bond arrays missing"));
+    std::vector<int> deg(N + 1, 0);
+    for (int64_t b = 0; b < cfg->n_bonds; ++b) {
+      int i = cfg->bond_i[b], j = cfg->bond_j[b];
+      if (i < 0 || j < 0 || (size_t)i >= N || (size_t)j >= N || i == j)
+        return bail(fail(RDS_E_ARG, "bond %lld has invalid particles (%d, %d)", (long long)b, i, j));
+      deg[i + 1]++;
+      deg[j + 1]++;
+    }
+    for (size_t i = 0; i < N; ++i) deg[i + 1] += deg[i];
+    std::vector<int> part(deg[N]), fill(deg.begin(), deg.end() - 1);
+    std::vector<double> r0v(deg[N]), u0v(deg[N]);
+    for (int64_t b = 0; b < cfg->n_bonds; ++b) {
+      int i = cfg->bond_i[b], j = cfg->bond_j[b];
+      int q = fill[i]++;
+      part[q] = j;
+      r0v[q] = cfg->bond_r0[b];
+      u0v[q] = cfg->bond_u0[b];
+      q = fill[j]++;
+      part[q] = i;
+      r0v[q] = cfg->bond_r0[b];
+      u0v[q] = cfg->bond_u0[b];
+    }
+    A_(c->boff, N + 1);
+    A_(c->bpart, part.size());
+    A_(c->br0, r0v.size());
+    A_(c->bu0, u0v.size());
+    A_(c->fbond, N);
+    HIP_TRY(hipMemcpy(c->boff, deg.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->bpart, part.data(), sizeof(int) * part.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->br0, r0v.data(), sizeof(double) * r0v.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->bu0, u0v.data(), sizeof(double) * u0v.size(), hipMemcpyHostToDevice));
+    c->any_bonds = true;
+  }
+#undef A_
+  *out = c;
+  return RDS_OK;
+}
+
+int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32_t* mol,
+                  int64_t step0) {
+  if (!c || !x || !types) return fail(RDS_E_ARG, "null argument");
+  if (c->has_mol && !mol) return fail(RDS_E_ARG, "molecule numbers required");
+  const int N = (int)c->n_total;
+  c->parity = 0;
+  Buf& a = c->buf[0];
+  Buf& b = c->buf[1];
+  hipLaunchKernelGGL(k_load, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, (const double2*)x,
+                     types, c->has_mol ? mol : nullptr, a.pos, a.type, a.pid,
+                     c->has_mol ? a.mol : nullptr);
+  hipLaunchKernelGGL(k_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep, c->d_rep_of,
+                     a.pos, c->ucell, c->urank, c->count);
+  hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
+                     c->cstart, c->d_step, 0);
+  hipLaunchKernelGGL(k_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos, c->ucell,
+                     c->urank, c->cstart, a.type, a.pid, a.mol, b.pos, b.type, b.pid, b.mol,
+                     c->slot_of);
+  c->parity = 1;
+  long long s0 = step0;
+  HIP_TRY(hipMemcpyAsync(c->d_step, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream));
+  DStatus st;
+  memset(&st, 0, sizeof st);
+  st.err_key = ~0ull;
+  HIP_TRY(hipMemcpyAsync(c->d_status, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipGetLastError());
+  c->loaded = true;
+  c->step0 = step0;
+  c->steps_done = 0;
+  RDS_TRY(build_graph(c));
+  return RDS_OK;
+}
+
+int rds_get_state(rds_ctx* c, double* x, int32_t* types) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
+  Buf& cur = c->buf[c->parity];
+  hipLaunchKernelGGL(k_gather_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                     (int)c->n_total, cur.pos, cur.type, cur.pid, (double2*)x, types);
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+int rds_set_injected(rds_ctx* c, const double* normals, const double* u1, const double* u2,
+                     int64_t first_step, int64_t n_steps) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  c->inj_n = normals;
+  c->inj_u1 = u1;
+  c->inj_u2 = u2;
+  c->inj_first = first_step;
+  c->inj_count = n_steps;
+  if (c->loaded) RDS_TRY(build_graph(c));
+  return RDS_OK;
+}
+
+int rds_set_random_force(rds_ctx* c, int32_t include) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  c->include_rf = include ? 1.0 : 0.0;
+  if (c->loaded) RDS_TRY(build_graph(c));
+  return RDS_OK;
+}
+
+int rds_run(rds_ctx* c, int64_t n_steps) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  if (!c->loaded) return fail(RDS_E_STATE, "run before set_state");
+  if (c->rng == RDS_RNG_INJECTED && !c->inj_n) return fail(RDS_E_STATE, "no injected streams set");
+  int64_t left = n_steps;
+  while (left > 0) {
+    if (c->graph_exec && left >= c->graph_len && c->parity == c->graph_parity) {
+      HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+      left -= c->graph_len;
+      c->steps_done += c->graph_len;
+    } else {
+      enqueue_step(c, nullptr);
+      left -= 1;
+      c->steps_done += 1;
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+int rds_sync(rds_ctx* c, rds_status* out) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  DStatus st;
+  HIP_TRY(hipMemcpy(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost));
+  rds_status s;
+  memset(&s, 0, sizeof s);
+  s.steps_done = c->steps_done;
+  s.bounds_error = st.bounds;
+  s.stream_error = st.stream;
+  s.reaction_events = (int64_t)st.events;
+  if (st.bounds) {
+    s.error_step = (int64_t)(st.err_key >> 32);
+    unsigned lo = (unsigned)(st.err_key & 0xffffffffu);
+    s.error_particle = (int32_t)(lo / 2);
+    s.error_axis = (int32_t)(lo % 2);
+    s.error_value = st.err_val;
+  }
+  if (out) *out = s;
+  if (st.bounds)
+    return fail(RDS_E_BOUNDS,
+                "x[%d,%d] = %.17g out of bounds. Maybe the timestep is too large for the given "
+                "the interaction parameters?",
+                s.error_particle, s.error_axis, s.error_value);
+  if (st.stream) return fail(RDS_E_STREAM, "injected random stream exhausted");
+  return RDS_OK;
+}
+
+int rds_compute_forces(rds_ctx* c, double* forces) {
+  if (!c || !forces) return fail(RDS_E_ARG, "null argument");
+  if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
+  Buf& cur = c->buf[c->parity];
+  if (c->any_bonds)
+    hipLaunchKernelGGL(k_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                       (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, c->slot_of,
+                       c->boff, c->bpart, c->br0, c->bu0, c->fbond);
+  StepArgs a = make_args(c, cur);
+  a.fout = (double2*)forces;
+  launch_force<MODE_FORCE>(c, a);
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+int rds_cell_ids(rds_ctx* c, int32_t* cells) {
+  if (!c || !cells) return fail(RDS_E_ARG, "null argument");
+  if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
+  Buf& cur = c->buf[c->parity];
+  hipLaunchKernelGGL(k_cells_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                     (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, cells);
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+int rds_grid(rds_ctx* c, int32_t* cells_xy, double* cell_len, int32_t* cell_base) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  for (int r = 0; r < c->n_rep; ++r) {
+    if (cells_xy) {
+      cells_xy[2 * r] = c->h_rep[r].lx;
+      cells_xy[2 * r + 1] = c->h_rep[r].ly;
+    }
+    if (cell_len) {
+      cell_len[2 * r] = c->h_rep[r].cl[0];
+      cell_len[2 * r + 1] = c->h_rep[r].cl[1];
+    }
+    if (cell_base) cell_base[r] = c->h_rep[r].cbase;
+  }
+  return RDS_OK;
+}
+
+int rds_neighbor_counts(rds_ctx* c, double r, int32_t* counts) {
+  if (!c || !counts) return fail(RDS_E_ARG, "null argument");
+  if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
+  for (int q = 0; q < c->n_rep; ++q) {
+    const DRep& d = c->h_rep[q];
+    if (r > std::min(d.cl[0], d.cl[1]))
+      return fail(RDS_E_ARG, "probe radius %g exceeds the cell length of replica %d", r, q);
+  }
+  Buf& cur = c->buf[c->parity];
+  StepArgs a = make_args(c, cur);
+  a.iout = counts;
+  a.rprobe2 = r * r;
+  launch_force<MODE_COUNT>(c, a);
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+int rds_time_kernels(rds_ctx* c, int64_t n_steps, double* avg_ms, int64_t* counts) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+  for (int64_t k = 0; k < n_steps; ++k) {
+    enqueue_step(c, &ev);
+    c->steps_done += 1;
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double sum[KT_N] = {0};
+  int64_t cnt[KT_N] = {0};
+  for (auto& e : ev) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e.second.first, e.second.second);
+    sum[e.first] += ms;
+    cnt[e.first] += 1;
+    hipEventDestroy(e.second.first);
+    hipEventDestroy(e.second.second);
+  }
+  for (int k = 0; k < KT_N; ++k) {
+    if (avg_ms) avg_ms[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+    if (counts) counts[k] = cnt[k];
+  }
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+int rds_sizes(rds_ctx* c, int64_t* n_total, int64_t* n_cells, int64_t* n_tiles) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  if (n_total) *n_total = c->n_total;
+  if (n_cells) *n_cells = c->n_cells;
+  if (n_tiles) *n_tiles = c->n_tiles;
+  return RDS_OK;
+}
+
+int rds_stream(rds_ctx* c, void** stream) {
+  if (!c || !stream) return fail(RDS_E_ARG, "null argument");
+  *stream = (void*)c->stream;
+  return RDS_OK;
+}
+
+}  // extern "C"

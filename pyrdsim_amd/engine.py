@@ -1,0 +1,296 @@
+"""Host side of the engine: model tables, replicas and device buffers.
+
+``Engine`` owns one ``rds_ctx`` of ``librds_hip.so``.  PyTorch-ROCm tensors
+serve only as device buffers for the arrays that cross the C ABI; all compute
+runs in the library's HIP kernels.
+"""
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+PRECISIONS = {'fp64': L.FP64, 'mixed': L.MIXED}
+RNGS = {'philox': L.RNG_PHILOX, 'injected': L.RNG_INJECTED}
+PROBS = {'linear': L.PROB_LINEAR, 'exp': L.PROB_EXP}
+
+
+def _lower(d):
+    return {str(k).lower(): v for k, v in d.items()}
+
+
+def build_model(interactions, reactions, n_types):
+    """Model table from the reference's interaction and reaction dicts.
+
+    ``interactions`` are the sections of nonbonded_interactions.txt (or the
+    ``interactions`` entry of the constructor dict), ``reactions`` those of
+    reactions.txt; both follow pyrdsim.pyx:1037-1154 and 2070-2222.  Returns
+    ``(model, rates)`` with the per-reaction rates in file order.
+    """
+    m = L.Model()
+    if not 1 <= n_types <= L.MAX_TYPES:
+        raise RuntimeError('the engine supports 1..%d particle types, got %d'
+                           % (L.MAX_TYPES, n_types))
+    m.n_types = n_types
+    nt = nw = 0
+    for raw in interactions:
+        it = _lower(raw)
+        kind = str(it['type']).lower()
+        if kind == 'wall':
+            if nw >= L.MAX_WALLS:
+                raise RuntimeError('too many wall interactions')
+            w = m.walls[nw]
+            w.particle_type = int(it['particle_type'])
+            w.direction = int(it['direction'])
+            w.u0 = float(it['u0'])
+            w.r0 = float(it['r0'])
+            w.a = float(it['a'])
+            if w.direction not in (0, 1):
+                raise RuntimeError('wall direction must be 0 or 1 in two dimensions')
+            nw += 1
+            continue
+        if kind not in L.PAIR_KINDS:
+            raise RuntimeError("Interaction {0} not recognized".format(kind))
+        if nt >= L.MAX_TERMS:
+            raise RuntimeError('too many pair interactions')
+        t = m.terms[nt]
+        t.kind = L.PAIR_KINDS[kind]
+        pair = [int(v) for v in np.asarray(it['pair']).ravel()]
+        t.type_a, t.type_b = pair[0], pair[1]
+        t.u0 = float(it['u0'])
+        t.r0 = float(it.get('r0', 0.0))
+        t.rc = float(it['r0'] if kind == 'piecewise_linear' else it['rc'])
+        t.n = int(it.get('n', 0))
+        t.a = float(it.get('a', 1.0))
+        t.dr = float(it.get('dr', 1.0))
+        nt += 1
+    m.n_terms = nt
+    m.n_walls = nw
+    rates = []
+    for q, raw in enumerate(reactions):
+        rx = _lower(raw)
+        if q >= L.MAX_REACTIONS:
+            raise RuntimeError('too many reactions')
+        kind = str(rx['type']).lower()
+        if kind not in L.REACTION_KINDS:
+            raise RuntimeError('reaction type {0} not recognized'.format(kind))
+        r = m.reactions[q]
+        r.kind = L.REACTION_KINDS[kind]
+        r.reactant = int(rx['reactant'])
+        r.product = int(rx['product'])
+        r.catalyst = int(rx.get('catalyst', -1))
+        r.inhibitor = int(rx.get('inhibitor', -1))
+        r.inhibitor2 = int(rx.get('inhibitor2', -1))
+        r.rate = float(rx['rate'])
+        r.r0 = float(rx.get('r0', 0.0))
+        r.rate_decrease = float(rx.get('rate_decrease', 0.0))
+        rates.append(r.rate)
+    m.n_reactions = len(rates)
+    return m, rates
+
+
+def prefactors(dt, tau_by_tauD):
+    """force_prefacs and noise_prefacs exactly as pyrdsim.pyx:488-496."""
+    tau = np.array(tau_by_tauD, dtype=np.float64)
+    mask = tau < 0
+    noise = np.ones_like(tau)
+    force = np.ones_like(tau)
+    noise[mask] = 0.
+    noise[~mask] = np.sqrt(2. * dt * tau[~mask])
+    force[mask] = 0.
+    force[~mask] = dt * tau[~mask]
+    return force, noise
+
+
+def make_replica(model_index, n_particles, L_box, periodic, dt, tau_by_tauD, rates, seed):
+    r = L.Replica()
+    r.model = model_index
+    r.n_particles = int(n_particles)
+    for k in range(2):
+        r.L[k] = float(L_box[k])
+        r.periodic[k] = 1 if float(periodic[k]) > 0 else 0
+    fp, npf = prefactors(dt, tau_by_tauD)
+    if len(fp) > L.MAX_TYPES:
+        raise RuntimeError('at most %d particle species' % L.MAX_TYPES)
+    for t in range(len(fp)):
+        r.force_prefac[t] = fp[t]
+        r.noise_prefac[t] = npf[t]
+    for q, v in enumerate(rates):
+        r.rate[q] = float(v)
+    r.dt = float(dt)
+    r.seed = int(seed) & 0xffffffffffffffff
+    return r
+
+
+class Engine:
+    """One engine context over one or more replicas on one GPU."""
+
+    def __init__(self, models, replicas, precision='mixed', rng='philox',
+                 reaction_prob='linear', bonds=None, has_molecules=False,
+                 tile_particles=0, graph_steps=0, device='cuda'):
+        self.lib = L.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError('the engine needs a ROCm GPU; none is visible')
+        self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device('cuda', torch.cuda.current_device())
+        torch.cuda.set_device(self.device)
+        self.models = (L.Model * len(models))(*models)
+        self.replicas = (L.Replica * len(replicas))(*replicas)
+        self.n_total = int(sum(r.n_particles for r in replicas))
+        self.has_molecules = bool(has_molecules)
+        cfg = L.Config()
+        cfg.n_models = len(models)
+        cfg.n_replicas = len(replicas)
+        cfg.models = C.cast(self.models, C.POINTER(L.Model))
+        cfg.replicas = C.cast(self.replicas, C.POINTER(L.Replica))
+        cfg.precision = PRECISIONS[precision]
+        cfg.rng = RNGS[rng]
+        cfg.reaction_prob = PROBS[reaction_prob]
+        cfg.has_molecules = 1 if has_molecules else 0
+        cfg.tile_particles = int(tile_particles)
+        cfg.cells_per_cutoff = 1
+        cfg.graph_steps = int(graph_steps)
+        self._bond_arrays = None
+        if bonds is not None and len(bonds[0]) > 0:
+            bi = np.ascontiguousarray(bonds[0], dtype=np.int32)
+            bj = np.ascontiguousarray(bonds[1], dtype=np.int32)
+            br = np.ascontiguousarray(bonds[2], dtype=np.float64)
+            bu = np.ascontiguousarray(bonds[3], dtype=np.float64)
+            self._bond_arrays = (bi, bj, br, bu)
+            cfg.n_bonds = len(bi)
+            cfg.bond_i = bi.ctypes.data_as(C.POINTER(C.c_int32))
+            cfg.bond_j = bj.ctypes.data_as(C.POINTER(C.c_int32))
+            cfg.bond_r0 = br.ctypes.data_as(C.POINTER(C.c_double))
+            cfg.bond_u0 = bu.ctypes.data_as(C.POINTER(C.c_double))
+        self.precision = precision
+        self.rng = rng
+        ctx = C.c_void_p()
+        L.check(self.lib.rds_ctx_create(C.byref(cfg), C.byref(ctx)))
+        self.ctx = ctx
+        self._inj = None
+        self.step = 0
+
+    def close(self):
+        if getattr(self, 'ctx', None):
+            self.lib.rds_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- buffers ----------------------------------------------------------
+    def _dev(self, arr, dtype):
+        t = torch.as_tensor(np.ascontiguousarray(arr), dtype=dtype)
+        return t.to(self.device, non_blocking=False).contiguous()
+
+    def set_state(self, x, types, molecules=None, step0=0):
+        """Positions [N][2] f64, types [N], molecule numbers [N] (caller order)."""
+        x = np.asarray(x, dtype=np.float64).reshape(self.n_total, 2)
+        types = np.asarray(types).reshape(-1)
+        lo = 0
+        for r in self.replicas:
+            nt = self.models[r.model].n_types
+            seg = types[lo:lo + r.n_particles]
+            if seg.size and (seg.min() < 0 or seg.max() >= nt):
+                raise RuntimeError('particle types must lie in [0, %d) for this model' % nt)
+            lo += r.n_particles
+        xd = self._dev(x, torch.float64)
+        td = self._dev(np.asarray(types, dtype=np.int32).reshape(-1), torch.int32)
+        md = None
+        if self.has_molecules:
+            md = self._dev(np.asarray(molecules, dtype=np.int32).reshape(-1), torch.int32)
+        torch.cuda.synchronize(self.device)
+        L.check(self.lib.rds_set_state(self.ctx, xd.data_ptr(), td.data_ptr(),
+                                       md.data_ptr() if md is not None else None,
+                                       int(step0)))
+        self.step = int(step0)
+
+    def set_injected(self, normals, u1, u2=None, first_step=1):
+        """Injected streams: normals [S][N][2], u1/u2 [S][N] for steps
+        first_step .. first_step+S-1 (the reference's step numbers)."""
+        n = self._dev(np.asarray(normals, dtype=np.float64), torch.float64)
+        a = self._dev(np.asarray(u1, dtype=np.float64), torch.float64)
+        b = self._dev(np.asarray(u2, dtype=np.float64), torch.float64) if u2 is not None else None
+        S = n.shape[0]
+        if n.numel() != S * self.n_total * 2 or a.numel() != S * self.n_total:
+            raise ValueError('injected stream shapes do not match the particle count')
+        torch.cuda.synchronize(self.device)
+        self._inj = (n, a, b)  # keep alive
+        L.check(self.lib.rds_set_injected(self.ctx, n.data_ptr(), a.data_ptr(),
+                                          b.data_ptr() if b is not None else None,
+                                          int(first_step), int(S)))
+
+    def set_random_force(self, include):
+        L.check(self.lib.rds_set_random_force(self.ctx, 1 if include else 0))
+
+    # -- stepping ---------------------------------------------------------
+    def run(self, n_steps):
+        L.check(self.lib.rds_run(self.ctx, int(n_steps)))
+        self.step += int(n_steps)
+
+    def sync(self):
+        """Wait for the GPU; returns the status, raises on a kernel-side error."""
+        st = L.Status()
+        rc = self.lib.rds_sync(self.ctx, C.byref(st))
+        if rc == L.E_BOUNDS:
+            raise RuntimeError(self.lib.rds_last_error().decode())
+        L.check(rc)
+        return st
+
+    # -- outputs ----------------------------------------------------------
+    def state(self):
+        x = torch.empty((self.n_total, 2), dtype=torch.float64, device=self.device)
+        t = torch.empty((self.n_total,), dtype=torch.int32, device=self.device)
+        L.check(self.lib.rds_get_state(self.ctx, x.data_ptr(), t.data_ptr()))
+        self.sync()
+        return x.cpu().numpy(), t.cpu().numpy()
+
+    def state_device(self, x, t):
+        """Write the state into caller-provided device tensors (no sync)."""
+        L.check(self.lib.rds_get_state(self.ctx, x.data_ptr(), t.data_ptr()))
+
+    def forces(self):
+        f = torch.empty((self.n_total, 2), dtype=torch.float64, device=self.device)
+        L.check(self.lib.rds_compute_forces(self.ctx, f.data_ptr()))
+        self.sync()
+        return f.cpu().numpy()
+
+    def cell_ids(self):
+        c = torch.empty((self.n_total,), dtype=torch.int32, device=self.device)
+        L.check(self.lib.rds_cell_ids(self.ctx, c.data_ptr()))
+        self.sync()
+        return c.cpu().numpy()
+
+    def grid(self):
+        n = len(self.replicas)
+        lxy = np.zeros((n, 2), dtype=np.int32)
+        cl = np.zeros((n, 2), dtype=np.float64)
+        base = np.zeros((n,), dtype=np.int32)
+        L.check(self.lib.rds_grid(self.ctx, lxy.ctypes.data, cl.ctypes.data, base.ctypes.data))
+        return lxy, cl, base
+
+    def neighbor_counts(self, r):
+        c = torch.empty((self.n_total,), dtype=torch.int32, device=self.device)
+        L.check(self.lib.rds_neighbor_counts(self.ctx, float(r), c.data_ptr()))
+        self.sync()
+        return c.cpu().numpy()
+
+    def time_kernels(self, n_steps):
+        """Average HIP-event duration (ms) of each kernel over n_steps eager steps."""
+        avg = np.zeros(len(L.KERNELS), dtype=np.float64)
+        cnt = np.zeros(len(L.KERNELS), dtype=np.int64)
+        L.check(self.lib.rds_time_kernels(self.ctx, int(n_steps), avg.ctypes.data,
+                                          cnt.ctypes.data))
+        self.step += int(n_steps)
+        return {k: (float(avg[i]), int(cnt[i])) for i, k in enumerate(L.KERNELS)}
+
+    def sizes(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        L.check(self.lib.rds_sizes(self.ctx, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
