@@ -45,13 +45,29 @@ def _make_replica_state(args):
     return s['x'], s['types']
 
 
-def build_ensemble(n_rep, workers):
+def build_ensemble(n_rep, workers, cache=None):
+    """Replica plan and initial states; `cache` (an .npz path) is read when it
+    exists and written otherwise, so that profiled runs start without forking
+    worker processes."""
     plan = workloads.ensemble_plan(n_rep=n_rep)
+    if cache and os.path.exists(cache):
+        with np.load(cache, allow_pickle=False) as z:
+            x, t, n = z['x'], z['t'], z['n']
+        if len(n) == len(plan) and all(int(a) == p['n'] for a, p in zip(n, plan)):
+            off = np.concatenate([[0], np.cumsum(n)])
+            return plan, [(x[off[r]:off[r + 1]], t[off[r]:off[r + 1]]) for r in range(len(plan))]
     jobs = [(p['kind'], p['n'], p['k_a'], p['k_d'], p['seed'], 1005 * 1000 + r)
             for r, p in enumerate(plan)]
-    ctx = mp.get_context('fork')
-    with ctx.Pool(workers) as pool:
-        states = pool.map(_make_replica_state, jobs, chunksize=4)
+    if workers > 1:
+        ctx = mp.get_context('fork')
+        with ctx.Pool(workers) as pool:
+            states = pool.map(_make_replica_state, jobs, chunksize=4)
+    else:
+        states = [_make_replica_state(j) for j in jobs]
+    if cache:
+        np.savez(cache, x=np.concatenate([s[0] for s in states]),
+                 t=np.concatenate([s[1] for s in states]),
+                 n=np.array([p['n'] for p in plan]))
     return plan, states
 
 
@@ -133,6 +149,9 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
     ap.add_argument('--graph-steps', type=int, default=50)
     ap.add_argument('--tile-particles', type=int, default=0)
+    ap.add_argument('--tile-rows', type=int, default=1)
+    ap.add_argument('--lists', action='store_true')
+    ap.add_argument('--ic-cache', default=None, help='.npz file of initial states (read or written)')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     args = ap.parse_args()
@@ -142,7 +161,7 @@ def main():
         sys.exit(2)
 
     t0 = time.perf_counter()
-    plan, states = build_ensemble(args.replicas, min(16, os.cpu_count() or 1))
+    plan, states = build_ensemble(args.replicas, min(16, os.cpu_count() or 1), args.ic_cache)
     print('# built %d replica states in %.1f s' % (len(plan), time.perf_counter() - t0),
           file=sys.stderr, flush=True)
 
@@ -154,7 +173,8 @@ def main():
     t0s = np.concatenate([s[1] for s in states])
 
     eng = Engine(models, reps, precision='mixed', graph_steps=args.graph_steps,
-                 tile_particles=args.tile_particles)
+                 tile_particles=args.tile_particles, tile_rows=args.tile_rows,
+                 lists=args.lists)
     eng.set_state(x0, t0s)
     secs = time_engine(eng, args.steps, args.warmup)
     value = n_tot * args.steps / secs
@@ -185,7 +205,8 @@ def main():
         tsum = 0.0
         for r in pick:
             e1 = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64',
-                        graph_steps=args.graph_steps)
+                        graph_steps=args.graph_steps, tile_particles=args.tile_particles,
+                        tile_rows=args.tile_rows)
             e1.set_state(states[r][0], states[r][1])
             dt = time_engine(e1, args.plain_steps, 20)
             work += plan[r]['n'] * args.plain_steps
@@ -219,6 +240,7 @@ def main():
                         'wall+reactions rho=0.4 dt=1e-4), N log-uniform 2048..32768, '
                         'sum N=%d' % (len(plan), n_tot),
             'precision': 'mixed', 'graph_steps': args.graph_steps,
+            'tile_rows': args.tile_rows, 'tile_particles': args.tile_particles,
             'cells': int(n_cells), 'tiles': int(n_tiles)},
         'roofline': roofline,
         'plain': plain,

@@ -119,7 +119,7 @@ typedef struct {
   int32_t tile_particles;       /* target particles per force tile, 0 = 56 */
   int32_t cells_per_cutoff;     /* 1 -> 3x3 stencil, 2 -> 5x5; 0 = 1 */
   int32_t graph_steps;          /* steps per captured hipGraph, 0 = no graph */
-  int32_t reserved;
+  int32_t tile_rows;            /* cell rows per force/reaction tile: 1, 2, 4 (0 = 1) */
   /* harmonic bonds (pyrdsim.pyx:1303-1320, 2037-2047), global particle
    * indices, host arrays; n_bonds may be 0 */
   int64_t n_bonds;
@@ -139,6 +139,8 @@ typedef struct {
   int32_t error_axis;
   double error_value;       /* coordinate after the boundary conditions */
   int64_t reaction_events;  /* type changes since set_state */
+  int32_t integrity;        /* nonzero: an internal cell-list invariant failed */
+  int32_t reserved;
 } rds_status;
 
 typedef struct rds_ctx rds_ctx;
@@ -198,6 +200,11 @@ int rds_neighbor_counts(rds_ctx* ctx, double r, int32_t* counts);
  * context's stream; avg_ms[k] for k = 0 force+integrate, 1 scan, 2 scatter,
  * 3 reaction, 4 bonds; counts[k] = launches timed. */
 int rds_time_kernels(rds_ctx* ctx, int64_t n_steps, double* avg_ms, int64_t* counts);
+
+/* Test hook with no counterpart in the reference: fills the LDS of many
+ * workgroups with a 32-bit pattern, so that a following launch that read
+ * LDS it never wrote would see that pattern (NaN bits make it visible). */
+int rds_debug_fill_lds(rds_ctx* ctx, uint32_t pattern);
 
 /* Sizes: n_total particles, n_cells, n_tiles. */
 int rds_sizes(rds_ctx* ctx, int64_t* n_total, int64_t* n_cells, int64_t* n_tiles);

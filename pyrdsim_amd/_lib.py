@@ -72,7 +72,7 @@ class Config(C.Structure):
                 ('precision', C.c_int32), ('rng', C.c_int32),
                 ('reaction_prob', C.c_int32), ('has_molecules', C.c_int32),
                 ('tile_particles', C.c_int32), ('cells_per_cutoff', C.c_int32),
-                ('graph_steps', C.c_int32), ('reserved', C.c_int32),
+                ('graph_steps', C.c_int32), ('tile_rows', C.c_int32),
                 ('n_bonds', C.c_int64), ('bond_i', C.POINTER(C.c_int32)),
                 ('bond_j', C.POINTER(C.c_int32)), ('bond_r0', C.POINTER(C.c_double)),
                 ('bond_u0', C.POINTER(C.c_double))]
@@ -82,7 +82,8 @@ class Status(C.Structure):
     _fields_ = [('steps_done', C.c_int64), ('bounds_error', C.c_int32),
                 ('stream_error', C.c_int32), ('error_step', C.c_int64),
                 ('error_particle', C.c_int32), ('error_axis', C.c_int32),
-                ('error_value', C.c_double), ('reaction_events', C.c_int64)]
+                ('error_value', C.c_double), ('reaction_events', C.c_int64),
+                ('integrity', C.c_int32), ('reserved', C.c_int32)]
 
 
 # every entry point of include/rds.h: name -> (restype, argtypes)
@@ -103,6 +104,7 @@ SIGNATURES = {
     'rds_grid': (C.c_int, [_VP, _VP, _VP, _VP]),
     'rds_neighbor_counts': (C.c_int, [_VP, C.c_double, _VP]),
     'rds_time_kernels': (C.c_int, [_VP, C.c_int64, _VP, _VP]),
+    'rds_debug_fill_lds': (C.c_int, [_VP, C.c_uint32]),
     'rds_sizes': (C.c_int, [_VP, _VP, _VP, _VP]),
     'rds_stream': (C.c_int, [_VP, C.POINTER(_VP)]),
 }

@@ -46,8 +46,7 @@
 #define MAXWALLS RDS_MAX_WALLS
 #define MAXRX RDS_MAX_REACTIONS
 #define MAXW 60        // cells per tile strip
-#define CAP 384        // staged particles per LDS round
-#define FBLOCK 64      // threads of a force / reaction workgroup (one wave)
+#define LMAX 40        // per-lane partner list of the force kernel
 
 namespace {
 
@@ -88,12 +87,14 @@ struct DWall {
 struct DRx {
   int kind, reactant, product, catalyst, inh, inh2, pad0, pad1;
   double r0, rate_decrease;
+  double r0sq_lo, r0sq_hi;  // r0^2 (1 -+ 1e-14): outside this band d2 decides d < r0
 };
 
 struct DModel {
   int nt, nterms, nwalls, nrx;
   int nbr_rx;      // any catalysed / inhibited reaction
   int pad0, pad1, pad2;
+  double rc2max;   // largest pair cutoff squared
   int pfirst[MAXT * MAXT];
   int pcount[MAXT * MAXT];
   DTerm terms[MAXTERMS];
@@ -116,6 +117,8 @@ struct DStatus {
   double err_val;
   int bounds, stream;
   unsigned long long events;
+  int integrity;               // bit 0: scatter index out of range, bit 1: histogram total
+  int pad;
 };
 
 // Everything a kernel of one step needs.  Passed by value.
@@ -153,6 +156,9 @@ struct StepArgs {
   DStatus* status;
   // reaction output
   int* type_out;
+  // staged particles per LDS round (dynamic shared memory)
+  int cap;
+  const int* rep_of;  // replica of each pid
 };
 
 // ---------------------------------------------------------------------------
@@ -229,133 +235,172 @@ struct Real2 {
 };
 
 // ---------------------------------------------------------------------------
-// tile staging shared by the force and reaction kernels
+// tiles: R cell rows x W cells.  The staged region is rows cy0-1 .. cy0+R and
+// cells cx0-1 .. cx0+W of each row, periodic images shifted by -L / +L.  Each
+// staged row is at most 3 contiguous slot segments (left image, in-range run,
+// right image).  Wave q of the workgroup owns the home particles of row q.
 // ---------------------------------------------------------------------------
 
-struct TileShared {
-  int off[3][MAXW + 4];   // staged offset of cell k of row r (prefix, k = 0..W+2)
-  int seg_slot[9];        // up to 3 contiguous slot segments per row
-  int seg_n[9];
-  int seg_off[9];         // staged offset of the segment
-  double seg_sx[9], seg_sy[9];
-  int nseg;
-  int total;
-  int home_begin, home_end;
-  int row_base[3];
-  int mid_slot1, mid_off1; // home row: first slot and staged offset of the
-                           // contiguous in-range segment
+template <int R>
+struct TileSh {
+  int cs[R + 2][MAXW + 2];   // first slot of staged cell k (k = 0..W+1)
+  int off[R + 2][MAXW + 3];  // staged offset of cell k within its row (prefix)
+  int row_base[R + 3];
+  int seg_slot[R + 2][3], seg_n[R + 2][3], seg_off[R + 2][3];
+  double seg_sx[R + 2][3];
+  double row_sy[R + 2];
+  int total, nbatch;
 };
 
-// Computes the staged layout of a tile: for the three rows cy-1, cy, cy+1 the
-// cells cx0-1 .. cx0+W (periodic images shifted by -L / +L).
-__device__ void tile_layout(const DRep& R, const int* __restrict__ cstart, int cy, int cx0,
-                            int W, TileShared& sh) {
-  const int lane = threadIdx.x;
-  const int lx = R.lx, ly = R.ly;
-  // per-cell counts
-  for (int r = 0; r < 3; ++r) {
-    int y = cy + r - 1;
+template <int R>
+__device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int cy0, int cx0,
+                           int W, int nrows, TileSh<R>& sh) {
+  const int t = threadIdx.x, nthr = blockDim.x;
+  const int NR = nrows + 2;
+  const int lx = Rp.lx, ly = Rp.ly;
+  for (int q = t; q < NR * (W + 2); q += nthr) {
+    int r = q / (W + 2), k = q - r * (W + 2);
+    int y = cy0 + r - 1;
+    bool ok = true;
+    if (y < 0) { if (Rp.sy) y += ly; else ok = false; }
+    else if (y >= ly) { if (Rp.sy) y -= ly; else ok = false; }
+    int x = cx0 - 1 + k;
+    if (x < 0) { if (Rp.sx) x += lx; else ok = false; }
+    else if (x >= lx) { if (Rp.sx) x -= lx; else ok = false; }
+    int cs = 0, n = 0;
+    if (ok) {
+      int c = Rp.cbase + y * lx + x;
+      cs = cstart[c];
+      n = cstart[c + 1] - cs;
+    }
+    sh.cs[r][k] = cs;
+    sh.off[r][k + 1] = n;
+  }
+  __syncthreads();
+  {
+    const int wv = t >> 6, lane = t & 63;
+    for (int r = wv; r < NR; r += (nthr >> 6)) {
+      int v = (lane < W + 2) ? sh.off[r][lane + 1] : 0;
+      int inc = v;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        int o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+      }
+      if (lane < W + 2) sh.off[r][lane + 1] = inc;
+      if (lane == 0) sh.off[r][0] = 0;
+    }
+  }
+  __syncthreads();
+  if (t < NR) {
+    const int r = t;
+    int y = cy0 + r - 1;
     bool rowok = true;
-    if (y < 0) { if (R.sy) y += ly; else rowok = false; }
-    if (y >= ly) { if (R.sy) y -= ly; else rowok = false; }
-    for (int k = lane; k < W + 2; k += FBLOCK) {
-      int x = cx0 - 1 + k;
-      bool ok = rowok;
-      if (x < 0) { if (R.sx) x += lx; else ok = false; }
-      if (x >= lx) { if (R.sx) x -= lx; else ok = false; }
-      int cnt = 0;
-      if (ok) {
-        int c = R.cbase + y * lx + x;
-        cnt = cstart[c + 1] - cstart[c];
+    double sy = 0.0;
+    if (y < 0) { if (Rp.sy) sy = -Rp.L[1]; else rowok = false; }
+    else if (y >= ly) { if (Rp.sy) sy = Rp.L[1]; else rowok = false; }
+    sh.row_sy[r] = sy;
+    for (int g = 0; g < 3; ++g) sh.seg_n[r][g] = 0;
+    if (rowok) {
+      if (cx0 == 0 && Rp.sx) {
+        sh.seg_slot[r][0] = sh.cs[r][0];
+        sh.seg_n[r][0] = sh.off[r][1];
+        sh.seg_off[r][0] = 0;
+        sh.seg_sx[r][0] = -Rp.L[0];
       }
-      sh.off[r][k + 1] = cnt;
+      int ka = (cx0 == 0) ? 1 : 0;
+      int kb = (cx0 + W >= lx) ? W : W + 1;
+      sh.seg_slot[r][1] = sh.cs[r][ka];
+      sh.seg_n[r][1] = sh.off[r][kb + 1] - sh.off[r][ka];
+      sh.seg_off[r][1] = sh.off[r][ka];
+      sh.seg_sx[r][1] = 0.0;
+      if (cx0 + W >= lx && Rp.sx) {
+        sh.seg_slot[r][2] = sh.cs[r][W + 1];
+        sh.seg_n[r][2] = sh.off[r][W + 2] - sh.off[r][W + 1];
+        sh.seg_off[r][2] = sh.off[r][W + 1];
+        sh.seg_sx[r][2] = Rp.L[0];
+      }
     }
   }
-  __syncthreads();
-  if (lane < 3) {
-    int r = lane;
-    int acc = 0;
-    sh.off[r][0] = 0;
-    for (int k = 1; k <= W + 2; ++k) {
-      acc += sh.off[r][k];
-      sh.off[r][k] = acc;
-    }
-  }
-  __syncthreads();
-  if (lane == 0) {
-    int base = 0, ns = 0;
-    for (int r = 0; r < 3; ++r) {
+  if (t == 0) {
+    int base = 0, nb = 0;
+    for (int r = 0; r < NR; ++r) {
       sh.row_base[r] = base;
-      int y = cy + r - 1;
-      double sy = 0.0;
-      bool rowok = true;
-      if (y < 0) { if (R.sy) { y += ly; sy = -R.L[1]; } else rowok = false; }
-      if (y >= ly) { if (R.sy) { y -= ly; sy = R.L[1]; } else rowok = false; }
-      if (rowok) {
-        int rowc = R.cbase + y * lx;
-        // left periodic image
-        if (cx0 - 1 < 0 && R.sx) {
-          int c = rowc + lx - 1;
-          sh.seg_slot[ns] = cstart[c];
-          sh.seg_n[ns] = cstart[c + 1] - cstart[c];
-          sh.seg_off[ns] = base + sh.off[r][0];
-          sh.seg_sx[ns] = -R.L[0];
-          sh.seg_sy[ns] = sy;
-          ++ns;
-        }
-        int xa = max(cx0 - 1, 0), xb = min(cx0 + W, lx - 1);
-        int ka = xa - (cx0 - 1);
-        sh.seg_slot[ns] = cstart[rowc + xa];
-        sh.seg_n[ns] = cstart[rowc + xb + 1] - cstart[rowc + xa];
-        sh.seg_off[ns] = base + sh.off[r][ka];
-        sh.seg_sx[ns] = 0.0;
-        sh.seg_sy[ns] = sy;
-        if (r == 1) {
-          sh.mid_slot1 = sh.seg_slot[ns];
-          sh.mid_off1 = sh.seg_off[ns];
-        }
-        ++ns;
-        if (cx0 + W >= lx && R.sx) {
-          int c = rowc;
-          sh.seg_slot[ns] = cstart[c];
-          sh.seg_n[ns] = cstart[c + 1] - cstart[c];
-          sh.seg_off[ns] = base + sh.off[r][W + 1];
-          sh.seg_sx[ns] = R.L[0];
-          sh.seg_sy[ns] = sy;
-          ++ns;
-        }
-      }
       base += sh.off[r][W + 2];
     }
-    sh.nseg = ns;
+    sh.row_base[NR] = base;
     sh.total = base;
-    int hc = R.cbase + cy * lx + cx0;
-    sh.home_begin = cstart[hc];
-    sh.home_end = cstart[hc + W];
+    for (int q = 0; q < nrows; ++q) {
+      int nh = sh.off[q + 1][W + 1] - sh.off[q + 1][1];
+      nb = max(nb, (nh + 63) >> 6);
+    }
+    sh.nbatch = nb;
   }
   __syncthreads();
 }
 
-// Stages staged indices [p0, p0 + CAP) into LDS.
-template <bool MOL>
-__device__ void tile_stage(const TileShared& sh, int p0, const double2* __restrict__ pos,
-                           const int* __restrict__ type, const int* __restrict__ mol,
-                           double2* s_pos, int* s_ty, int* s_mol) {
-  const int lane = threadIdx.x;
-  const int p1 = p0 + CAP;
-  for (int g = 0; g < sh.nseg; ++g) {
-    int so = sh.seg_off[g], sn = sh.seg_n[g];
-    int a = max(so, p0), b = min(so + sn, p1);
-    double sx = sh.seg_sx[g], sy = sh.seg_sy[g];
-    int sl = sh.seg_slot[g] - so;
-    for (int p = a + lane; p < b; p += FBLOCK) {
-      double2 q = pos[sl + p];
-      q.x = q.x + sx;
-      q.y = q.y + sy;
-      s_pos[p - p0] = q;
-      s_ty[p - p0] = type[sl + p];
-      if (MOL) s_mol[p - p0] = mol[sl + p];
+// Stages staged indices [p0, p0 + cap) into LDS.
+template <int R, bool MOL>
+__device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
+                           const double2* __restrict__ pos, const int* __restrict__ type,
+                           const int* __restrict__ mol, double2* s_pos, int* s_ty, int* s_mol) {
+  const int t = threadIdx.x, nthr = blockDim.x;
+  const int p1 = p0 + cap;
+  for (int r = 0; r < NR; ++r) {
+    const double sy = sh.row_sy[r];
+    for (int g = 0; g < 3; ++g) {
+      const int n = sh.seg_n[r][g];
+      if (n == 0) continue;
+      const int so = sh.row_base[r] + sh.seg_off[r][g];
+      const int a = max(so, p0), b = min(so + n, p1);
+      const double sx = sh.seg_sx[r][g];
+      const int sl = sh.seg_slot[r][g] - so;
+      for (int p = a + t; p < b; p += nthr) {
+        double2 q = pos[sl + p];
+        q.x = q.x + sx;
+        q.y = q.y + sy;
+        s_pos[p - p0] = q;
+        s_ty[p - p0] = type[sl + p];
+        if (MOL) s_mol[p - p0] = mol[sl + p];
+      }
     }
+  }
+}
+
+// Home particle of lane `lane` of wave q in batch b, and its three windows.
+template <int R>
+struct Home {
+  int s;
+  bool valid;
+  int w, own;
+  int lo[3], hi[3];
+};
+
+template <int R>
+__device__ __forceinline__ void home_windows(const TileSh<R>& sh, int q, int b, int lane,
+                                             int nrows, const DRep& Rp, int cx0, int W,
+                                             const double2* __restrict__ pos, Home<R>& h,
+                                             double2& xi) {
+  const int hr = q + 1;  // staged row of the home row
+  const bool rowok = q < nrows;
+  const int hb = rowok ? sh.cs[hr][1] : 0;
+  const int he = rowok ? sh.cs[hr][1] + (sh.off[hr][W + 1] - sh.off[hr][1]) : 0;
+  h.s = hb + b * 64 + lane;
+  h.valid = rowok && h.s < he;
+  xi = make_double2(0.0, 0.0);
+  h.w = 0;
+  h.own = -1;
+  if (h.valid) {
+    xi = pos[h.s];
+    int cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
+    h.w = min(max(cx - cx0, 0), W - 1);
+    h.own = sh.row_base[hr] + sh.off[hr][h.w + 1] + (h.s - sh.cs[hr][h.w + 1]);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int r = q + d;
+    h.lo[d] = h.valid ? sh.row_base[r] + sh.off[r][h.w] : 0;
+    h.hi[d] = h.valid ? sh.row_base[r] + sh.off[r][h.w + 3] : 0;
   }
 }
 
@@ -442,109 +487,190 @@ __device__ __forceinline__ double wall_force(const DWall& w, double x) {
 }
 
 // ---------------------------------------------------------------------------
-// force (+ integrate + bin) kernel
+// force (+ integrate + bin) kernel.  FAST: every type pair of every model has
+// at most one term, all LJ with the same even exponent 2*pw, at most 2 types;
+// the pair constants then live in registers.  Otherwise the term table is in
+// LDS and each term goes through pair_coef.
 // ---------------------------------------------------------------------------
 
-enum { MODE_STEP = 0, MODE_FORCE = 1, MODE_COUNT = 2 };
+enum { MODE_STEP = 0, MODE_FORCE = 1 };
 
-template <typename Real, bool MOL, int MODE>
-__global__ __launch_bounds__(FBLOCK) void k_force(const StepArgs A) {
-  __shared__ TileShared sh;
-  __shared__ double2 s_pos[CAP];
-  __shared__ int s_ty[CAP];
-  __shared__ int s_mol[MOL ? CAP : 1];
-  __shared__ int s_pfirst[MAXT * MAXT], s_pcount[MAXT * MAXT];
-  __shared__ DTerm s_term[MAXTERMS];
+// dynamic LDS of a tile kernel: cap x (double2 position, int type [, int molecule])
+struct Staging {
+  double2* pos;
+  int* ty;
+  int* mol;
+};
+__device__ __forceinline__ Staging staging(char* smem, int cap) {
+  Staging g;
+  g.pos = (double2*)smem;
+  g.ty = (int*)(g.pos + cap);
+  g.mol = g.ty + cap;
+  return g;
+}
 
-  const int lane = threadIdx.x;
+// KM: 0 = generic term table; 1 = single LJ term per type pair, runtime power;
+// 2 = the same with n = 6 (power 3) unrolled.
+template <typename Real, int KM>
+__device__ __forceinline__ Real lj_fast(Real r2, Real pa, Real pb, int pw) {
+  const Real inv = rcp_(r2);
+  const Real bb = pb * inv;
+  Real s;
+  if (KM == 2) {
+    s = bb * bb * bb;
+  } else {
+    s = bb;
+    for (int k = 1; k < pw; ++k) s *= bb;
+  }
+  return pa * inv * s * (s - Real(1));
+}
+
+template <typename Real, bool MOL, int KM, int R, bool LISTS>
+__global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, int pw) {
+  constexpr bool FAST = KM != 0;
+  extern __shared__ __align__(16) char smem[];
+  const int CAPT = A.cap;
+  const Staging G = staging(smem, CAPT);
+  double2* s_pos = G.pos;
+  int* s_ty = G.ty;
+  int* s_mol = G.mol;
+  __shared__ TileSh<R> sh;
+  __shared__ int s_pfirst[FAST ? 1 : MAXT * MAXT], s_pcount[FAST ? 1 : MAXT * MAXT];
+  __shared__ DTerm s_term[FAST ? 1 : MAXTERMS];
+  __shared__ unsigned short s_list[LISTS ? R * 64 * LMAX : 1];  // per-lane partner lists
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
   const int tile = xcd_tile(blockIdx.x, A.n_tiles);
   const int4 T = A.tiles[tile];
-  const DRep& R = A.rep[T.x];
-  const DModel& M = A.models[R.model];
-  const int cy = T.y, cx0 = T.z, W = T.w;
+  const DRep& Rp = A.rep[T.x & 0xffffff];
+  const int nrows = (T.x >> 24) + 1;
+  const DModel& M = A.models[Rp.model];
+  const int cy0 = T.y, cx0 = T.z, W = T.w;
   const int nt = M.nt;
+  const int NR = nrows + 2;
 
-  for (int q = lane; q < nt * nt; q += FBLOCK) {
-    s_pfirst[q] = M.pfirst[q];
-    s_pcount[q] = M.pcount[q];
-  }
-  for (int q = lane; q < M.nterms; q += FBLOCK) s_term[q] = M.terms[q];
-  tile_layout(R, A.cstart, cy, cx0, W, sh);
-
-  const int total = sh.total;
-  const bool resident = total <= CAP;
-  if (resident) {
-    tile_stage<MOL>(sh, 0, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
-  }
-  __syncthreads();
-
-  long long step_next = 0;
-  if (MODE == MODE_STEP) step_next = *A.step + 1;
-
-  for (int hb = sh.home_begin; hb < sh.home_end; hb += FBLOCK) {
-    const int s = hb + lane;
-    const bool valid = s < sh.home_end;
-    double2 xi = make_double2(0.0, 0.0);
-    int ti = 0, mi = 0, w = 0, own = -1;
-    if (valid) {
-      xi = A.pos[s];
-      ti = A.type[s];
-      if (MOL) mi = A.mol[s];
-      int cx = cell_coord(xi.x, R.cl[0], R.lx);
-      w = cx - cx0;
-      w = min(max(w, 0), W - 1);
-      own = sh.mid_off1 + (s - sh.mid_slot1);
+  if (!FAST) {
+    for (int q = t; q < nt * nt; q += blockDim.x) {
+      s_pfirst[q] = M.pfirst[q];
+      s_pcount[q] = M.pcount[q];
     }
-    int wlo[3], whi[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      wlo[r] = valid ? sh.row_base[r] + sh.off[r][w] : 0;
-      whi[r] = valid ? sh.row_base[r] + sh.off[r][w + 3] : 0;
+    for (int q = t; q < M.nterms; q += blockDim.x) s_term[q] = M.terms[q];
+  }
+  tile_setup<R>(Rp, A.cstart, cy0, cx0, W, nrows, sh);
+  const int total = sh.total;
+  const bool resident = total <= CAPT;
+  if (resident) {
+    tile_stage<R, MOL>(sh, NR, 0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+    __syncthreads();
+  }
+  long long step_next = 0;
+  if (mode == MODE_STEP) step_next = *A.step + 1;
+
+  for (int b = 0; b < sh.nbatch; ++b) {
+    Home<R> h;
+    double2 xi;
+    home_windows<R>(sh, wv, b, lane, nrows, Rp, cx0, W, A.pos, h, xi);
+    int ti = 0, mi = 0, p = 0;
+    if (h.valid) {
+      ti = A.type[h.s];
+      p = A.pid[h.s];
+      if (MOL) mi = A.mol[h.s];
+    }
+    // FAST: the (ti, tj) constants for tj = 0, 1 in registers
+    Real pa0 = 0, pb0 = 0, pc0 = -1, pa1 = 0, pb1 = 0, pc1 = -1;
+    if (FAST) {
+      int q0 = ti * nt;
+      if (M.pcount[q0] > 0) {
+        const DTerm& tm = M.terms[M.pfirst[q0]];
+        pa0 = TP<Real>::c_lj(tm); pb0 = TP<Real>::r0sq(tm); pc0 = TP<Real>::rc2(tm);
+      }
+      if (nt > 1 && M.pcount[q0 + 1] > 0) {
+        const DTerm& tm = M.terms[M.pfirst[q0 + 1]];
+        pa1 = TP<Real>::c_lj(tm); pb1 = TP<Real>::r0sq(tm); pc1 = TP<Real>::rc2(tm);
+      }
     }
     Real fx = 0, fy = 0;
-    int ncount = 0;
     const int tbase = ti * nt;
-
-    for (int p0 = 0; p0 < total; p0 += CAP) {
+    // one candidate: staged index j, counted when `live`
+    auto pair = [&](int j, int p0, bool live) {
+      const double2 pj = s_pos[j];
+      const int tj = s_ty[j];
+      const Real dx = (Real)(pj.x - xi.x), dy = (Real)(pj.y - xi.y);
+      const Real r2 = dx * dx + dy * dy;
+      // self / same molecule: pairs of one molecule never interact; a
+      // particle's only zero-distance partner in an atomic model is itself
+      const bool other = MOL ? (s_mol[j] != mi) : (r2 > Real(0));
+      if (FAST) {
+        const bool m1 = tj == 1;
+        const Real pa = m1 ? pa1 : pa0, pb = m1 ? pb1 : pb0, pc = m1 ? pc1 : pc0;
+        Real c = lj_fast<Real, KM>(r2, pa, pb, pw);
+        c = (live && other && r2 <= pc) ? c : Real(0);
+        fx += c * dx;
+        fy += c * dy;
+      } else {
+        const int pq = tbase + tj;
+        const int q0 = s_pfirst[pq], qn = live ? s_pcount[pq] : 0;
+        for (int q = q0; q < q0 + qn; ++q) {
+          const DTerm& tm = s_term[q];
+          Real c = pair_coef<Real>(tm, r2);
+          c = (other && r2 <= TP<Real>::rc2(tm)) ? c : Real(0);
+          fx += c * dx;
+          fy += c * dy;
+        }
+      }
+    };
+    const double rc2max = M.rc2max;
+    unsigned short* my_list = s_list + wv * 64 * LMAX + lane;  // entry k at my_list[64 k]
+    for (int p0 = 0; p0 < total; p0 += CAPT) {
       if (!resident) {
         __syncthreads();
-        tile_stage<MOL>(sh, p0, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+        tile_stage<R, MOL>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
         __syncthreads();
       }
+      if (!LISTS) {
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        int lo = max(wlo[r], p0) - p0;
-        int hi = min(whi[r], p0 + CAP) - p0;
+        for (int d = 0; d < 3; ++d) {
+          const int lo = max(h.lo[d], p0) - p0;
+          const int hi = min(h.hi[d], p0 + CAPT) - p0;
+          // two candidates per trip; past the window end the second one
+          // repeats j with its contribution masked (it must read a staged
+          // element: 0 * dx of an unstaged one could be 0 * NaN)
+          for (int j = lo; j < hi; j += 2) {
+            pair(j, p0, true);
+            const bool two = j + 1 < hi;
+            pair(two ? j + 1 : j, p0, two);
+          }
+        }
+        continue;
+      }
+      // pass A: FP64 distance test over the 3x3 stencil; partners within the
+      // model's largest cutoff go to this lane's list (in window order)
+      int cnt = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int lo = max(h.lo[d], p0) - p0;
+        const int hi = min(h.hi[d], p0 + CAPT) - p0;
         for (int j = lo; j < hi; ++j) {
-          double2 pj = s_pos[j];
-          double dxd = pj.x - xi.x, dyd = pj.y - xi.y;
-          bool other = MOL ? (s_mol[j] != mi) : (j + p0 != own);
-          if (MODE == MODE_COUNT) {
-            double d2 = dxd * dxd + dyd * dyd;
-            ncount += (other && d2 <= A.rprobe2) ? 1 : 0;
-          } else {
-            Real dx = (Real)dxd, dy = (Real)dyd;
-            Real r2 = dx * dx + dy * dy;
-            int pq = tbase + s_ty[j];
-            int q0 = s_pfirst[pq], qn = s_pcount[pq];
-            for (int q = q0; q < q0 + qn; ++q) {
-              const DTerm& t = s_term[q];
-              Real c = pair_coef<Real>(t, r2);
-              bool in = other && (r2 <= TP<Real>::rc2(t));
-              c = in ? c : Real(0);
-              fx += c * dx;
-              fy += c * dy;
+          const double2 pj = s_pos[j];
+          const double ddx = pj.x - xi.x, ddy = pj.y - xi.y;
+          const double r2 = ddx * ddx + ddy * ddy;
+          const bool other = MOL ? (s_mol[j] != mi) : (r2 > 0.0);
+          if (other && r2 <= rc2max) {
+            if (cnt < LMAX) {
+              my_list[64 * cnt] = (unsigned short)j;
+              ++cnt;
+            } else {
+              pair(j, p0, true);  // list full: evaluate in place
             }
           }
         }
       }
+      // pass B: the pair laws over the list only
+      for (int k = 0; k < cnt; ++k) pair(my_list[64 * k], p0, true);
     }
-
-    if (!valid) continue;
-    if (MODE == MODE_COUNT) {
-      A.iout[A.pid[s]] = ncount;
-      continue;
-    }
+    if (!h.valid) continue;
+    const int s = h.s;
     double Fx = (double)fx, Fy = (double)fy;
     for (int q = 0; q < M.nwalls; ++q) {
       const DWall& wl = M.walls[q];
@@ -557,12 +683,10 @@ __global__ __launch_bounds__(FBLOCK) void k_force(const StepArgs A) {
       Fx += fb.x;
       Fy += fb.y;
     }
-    const int p = A.pid[s];
-    if (MODE == MODE_FORCE) {
+    if (mode == MODE_FORCE) {
       A.fout[p] = make_double2(Fx, Fy);
       continue;
     }
-
     // Euler-Maruyama, pyrdsim.pyx:2244-2250: two separate FP64 additions.
     double n0, n1;
     if (A.rng == RDS_RNG_INJECTED) {
@@ -577,12 +701,12 @@ __global__ __launch_bounds__(FBLOCK) void k_force(const StepArgs A) {
       }
     } else {
       float a0, a1;
-      philox_normals(R, p - R.pbase, step_next, a0, a1);
+      philox_normals(Rp, p - Rp.pbase, step_next, a0, a1);
       n0 = a0;
       n1 = a1;
     }
-    const double fp = R.fpre[ti];
-    const double np = A.include_rf * R.npre[ti];
+    const double fp = Rp.fpre[ti];
+    const double np = A.include_rf * Rp.npre[ti];
     double x = xi.x, y = xi.y;
     {
 #pragma clang fp contract(off)
@@ -592,14 +716,12 @@ __global__ __launch_bounds__(FBLOCK) void k_force(const StepArgs A) {
       y = y + np * n1;
     }
     // boundary conditions, pyrdsim.pyx:2269-2292
-    double Ls[2] = {R.L[0], R.L[1]};
-    int per[2] = {R.px, R.py};
     double v[2] = {x, y};
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      double L = Ls[k];
+      const double L = Rp.L[k];
       double u = v[k];
-      if (!per[k]) {
+      if (!(k == 0 ? Rp.px : Rp.py)) {
         if (u > L) u -= 2.0 * (u - L);
         if (u < 0.0) u -= 2.0 * u;
       } else {
@@ -617,100 +739,128 @@ __global__ __launch_bounds__(FBLOCK) void k_force(const StepArgs A) {
       }
       v[k] = u;
     }
-    int cx = cell_coord(v[0], R.cl[0], R.lx);
-    int cyy = cell_coord(v[1], R.cl[1], R.ly);
-    int c = R.cbase + cyy * R.lx + cx;
+    const int cx = cell_coord(v[0], Rp.cl[0], Rp.lx);
+    const int cyy = cell_coord(v[1], Rp.cl[1], Rp.ly);
+    const int c = Rp.cbase + cyy * Rp.lx + cx;
     A.upos[s] = make_double2(v[0], v[1]);
     A.ucell[s] = c;
     A.urank[s] = atomicAdd(&A.count[c], 1);
   }
 }
 
+// Neighbour counts within rprobe (diagnostic; FP64 distances).
+template <bool MOL>
+__global__ __launch_bounds__(64) void k_count(const StepArgs A) {
+  extern __shared__ __align__(16) char smem[];
+  const int CAPT = A.cap;
+  const Staging G = staging(smem, CAPT);
+  double2* s_pos = G.pos;
+  int* s_ty = G.ty;
+  int* s_mol = G.mol;
+  __shared__ TileSh<1> sh;
+  const int lane = threadIdx.x;
+  const int4 T = A.tiles[blockIdx.x];
+  const DRep& Rp = A.rep[T.x & 0xffffff];
+  const int nrows = (T.x >> 24) + 1;
+  const int W = T.w, cx0 = T.z;
+  // one home row at a time through a 1-row view of the tile
+  for (int q = 0; q < nrows; ++q) {
+    __syncthreads();
+    tile_setup<1>(Rp, A.cstart, T.y + q, cx0, W, 1, sh);
+    const int total = sh.total;
+    for (int b = 0; b < sh.nbatch; ++b) {
+      Home<1> h;
+      double2 xi;
+      home_windows<1>(sh, 0, b, lane, 1, Rp, cx0, W, A.pos, h, xi);
+      int mi = (MOL && h.valid) ? A.mol[h.s] : 0;
+      int cnt = 0;
+      for (int p0 = 0; p0 < total; p0 += CAPT) {
+        __syncthreads();
+        tile_stage<1, MOL>(sh, 3, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+        __syncthreads();
+        for (int d = 0; d < 3; ++d) {
+          const int lo = max(h.lo[d], p0) - p0;
+          const int hi = min(h.hi[d], p0 + CAPT) - p0;
+          for (int j = lo; j < hi; ++j) {
+            const bool other = MOL ? (s_mol[j] != mi) : (j + p0 != h.own);
+            const double dx = s_pos[j].x - xi.x, dy = s_pos[j].y - xi.y;
+            cnt += (other && dx * dx + dy * dy <= A.rprobe2) ? 1 : 0;
+          }
+        }
+      }
+      if (h.valid) A.iout[A.pid[h.s]] = cnt;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
-// reaction kernel (pyrdsim.pyx:2070-2222), exact reference arithmetic: rates are
-// accumulated by repeated addition in reaction order, as the reference does.
+// reaction kernel (pyrdsim.pyx:2070-2222), one thread per slot.  Exact
+// reference arithmetic: the pair distance is the reference's minimum image
+// dx - L*round(dx/L) and sqrt (pyrdsim.pyx:1628-1640), compared with d < r0;
+// rates are accumulated by repeated addition in reaction order, as the
+// reference does.  Only reactants of a neighbour-dependent reaction walk their
+// 3x3 cells (reading neighbours straight from the sorted arrays).
 // ---------------------------------------------------------------------------
 
 template <bool MOL>
-__global__ __launch_bounds__(FBLOCK) void k_react(const StepArgs A) {
-  __shared__ TileShared sh;
-  __shared__ double2 s_pos[CAP];
-  __shared__ int s_ty[CAP];
-  __shared__ int s_mol[MOL ? CAP : 1];
-
-  const int lane = threadIdx.x;
-  const int tile = xcd_tile(blockIdx.x, A.n_tiles);
-  const int4 T = A.tiles[tile];
-  const DRep& R = A.rep[T.x];
-  const DModel& M = A.models[R.model];
-  const int cy = T.y, cx0 = T.z, W = T.w;
+__global__ __launch_bounds__(256) void k_react(const StepArgs A) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.n_total) return;
+  const int p = A.pid[s];
+  const DRep& Rp = A.rep[A.rep_of[p]];
+  const DModel& M = A.models[Rp.model];
   const int nrx = M.nrx;
-  const long long step = *A.step;  // already advanced: the reference's l+1
-
-  tile_layout(R, A.cstart, cy, cx0, W, sh);
-  const int total = sh.total;
-  const bool need_nbr = M.nbr_rx != 0;
-  const bool resident = total <= CAP;
-  if (need_nbr && resident) tile_stage<MOL>(sh, 0, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
-  __syncthreads();
-
-  for (int hb = sh.home_begin; hb < sh.home_end; hb += FBLOCK) {
-    const int s = hb + lane;
-    const bool valid = s < sh.home_end;
-    double2 xi = make_double2(0.0, 0.0);
-    int ti = -1, mi = 0, w = 0, own = -1;
-    bool counts_needed = false;
-    if (valid) {
-      xi = A.pos[s];
-      ti = A.type[s];
-      if (MOL) mi = A.mol[s];
-      int cx = cell_coord(xi.x, R.cl[0], R.lx);
-      w = min(max(cx - cx0, 0), W - 1);
-      own = sh.mid_off1 + (s - sh.mid_slot1);
-      for (int q = 0; q < nrx; ++q)
-        if (M.rx[q].kind != RDS_RX_LINEAR && M.rx[q].reactant == ti) counts_needed = true;
-    }
+  const int ti = A.type[s];
+  int tnew = ti;
+  if (nrx > 0) {
+    const long long step = *A.step;  // already advanced: the reference's l+1
+    bool need = false;
+    for (int q = 0; q < nrx; ++q)
+      if (M.rx[q].kind != RDS_RX_LINEAR && M.rx[q].reactant == ti) need = true;
     int cnt[MAXRX];
 #pragma unroll
     for (int q = 0; q < MAXRX; ++q) cnt[q] = 0;
-    if (need_nbr) {
-      int wlo[3], whi[3];
-#pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        wlo[r] = counts_needed ? sh.row_base[r] + sh.off[r][w] : 0;
-        whi[r] = counts_needed ? sh.row_base[r] + sh.off[r][w + 3] : 0;
-      }
-      for (int p0 = 0; p0 < total; p0 += CAP) {
-        if (!resident) {
-          __syncthreads();
-          tile_stage<MOL>(sh, p0, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
-          __syncthreads();
+    if (need) {
+      const double2 xi = A.pos[s];
+      const int mi = MOL ? A.mol[s] : 0;
+      const int cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
+      const int cy = cell_coord(xi.y, Rp.cl[1], Rp.ly);
+      for (int oy = -1; oy <= 1; ++oy) {
+        int y = cy + oy;
+        if (y < 0 || y >= Rp.ly) {
+          if (!Rp.sy) continue;
+          y = (y < 0) ? y + Rp.ly : y - Rp.ly;
         }
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          int lo = max(wlo[r], p0) - p0;
-          int hi = min(whi[r], p0 + CAP) - p0;
-          for (int j = lo; j < hi; ++j) {
-            bool other = MOL ? (s_mol[j] != mi) : (j + p0 != own);
-            if (!other) continue;
-            double2 pj = s_pos[j];
-            double dx = pj.x - xi.x, dy = pj.y - xi.y;
-            double d = sqrt(dx * dx + dy * dy);
-            int tj = s_ty[j];
+        for (int ox = -1; ox <= 1; ++ox) {
+          int x = cx + ox;
+          if (x < 0 || x >= Rp.lx) {
+            if (!Rp.sx) continue;
+            x = (x < 0) ? x + Rp.lx : x - Rp.lx;
+          }
+          const int c = Rp.cbase + y * Rp.lx + x;
+          const int j1 = A.cstart[c + 1];
+          for (int j = A.cstart[c]; j < j1; ++j) {
+            if (j == s) continue;
+            if (MOL && A.mol[j] == mi) continue;
+            const double2 xj = A.pos[j];
+            double dx = xj.x - xi.x, dy = xj.y - xi.y;
+            if (Rp.px) dx = dx - Rp.L[0] * round(dx / Rp.L[0]);
+            if (Rp.py) dy = dy - Rp.L[1] * round(dy / Rp.L[1]);
+            const double d = sqrt(dx * dx + dy * dy);
+            const int tj = A.type[j];
 #pragma unroll
             for (int q = 0; q < MAXRX; ++q) {
               if (q >= nrx) break;
               const DRx& rx = M.rx[q];
               if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
-              bool match = (rx.kind == RDS_RX_CATALYZED) ? (tj == rx.catalyst)
-                                                         : (tj == rx.inh || tj == rx.inh2);
+              const bool match = (rx.kind == RDS_RX_CATALYZED) ? (tj == rx.catalyst)
+                                                               : (tj == rx.inh || tj == rx.inh2);
               if (match && d < rx.r0) cnt[q] += 1;
             }
           }
         }
       }
     }
-    if (!valid) continue;
     // rates in reaction order (pyrdsim.pyx:2103-2206)
     double rates[MAXRX];
     double total_rate = 0.0;
@@ -720,7 +870,7 @@ __global__ __launch_bounds__(FBLOCK) void k_react(const StepArgs A) {
     for (int q = 0; q < MAXRX; ++q) {
       if (q >= nrx) break;
       const DRx& rx = M.rx[q];
-      const double rate = R.rate[q];
+      const double rate = Rp.rate[q];
       if (rx.reactant != ti) continue;
       if (rx.kind == RDS_RX_LINEAR) {
         rates[q] = rate;
@@ -740,9 +890,7 @@ __global__ __launch_bounds__(FBLOCK) void k_react(const StepArgs A) {
         total_rate += acc;
       }
     }
-    int tnew = ti;
     if (total_rate > 0.0) {
-      const int p = A.pid[s];
       double u1, u2;
       if (A.rng == RDS_RNG_INJECTED) {
         long long o = step - A.inj_first;
@@ -755,10 +903,10 @@ __global__ __launch_bounds__(FBLOCK) void k_react(const StepArgs A) {
           A.status->stream = 1;
         }
       } else {
-        philox_uniforms(R, p - R.pbase, step, u1, u2);
+        philox_uniforms(Rp, p - Rp.pbase, step, u1, u2);
       }
-      double prob = (A.prob_mode == RDS_PROB_EXP) ? -expm1(-R.dt * total_rate)
-                                                  : R.dt * total_rate;
+      double prob = (A.prob_mode == RDS_PROB_EXP) ? -expm1(-Rp.dt * total_rate)
+                                                  : Rp.dt * total_rate;
       if (prob > u1) {
         double sacc = 0.0;
         int chosen = -1;
@@ -774,8 +922,8 @@ __global__ __launch_bounds__(FBLOCK) void k_react(const StepArgs A) {
         }
       }
     }
-    A.type_out[s] = tnew;
   }
+  A.type_out[s] = tnew;
 }
 
 // ---------------------------------------------------------------------------
@@ -830,7 +978,8 @@ __global__ void k_bin(int n_total, const DRep* __restrict__ rep, const int* __re
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const DRep* __restrict__ rep,
                                                      int* __restrict__ count,
                                                      int* __restrict__ cstart,
-                                                     long long* step, int advance) {
+                                                     long long* step, int advance,
+                                                     DStatus* status) {
   __shared__ int s_wsum[SCAN_BLOCK / 64];
   __shared__ int s_total;
   const DRep& R = rep[blockIdx.x];
@@ -879,6 +1028,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const DRep* __restrict__ re
     running += s_total;
     __syncthreads();
   }
+  if (t == 0 && running != R.pbase + R.n) atomicOr(&status->integrity, 2);
   if (advance && blockIdx.x == 0 && t == 0) *step = *step + 1;
 }
 
@@ -888,10 +1038,19 @@ __global__ void k_scatter(int n_total, const double2* __restrict__ upos,
                           const int* __restrict__ pid, const int* __restrict__ mol,
                           double2* __restrict__ npos, int* __restrict__ ntype,
                           int* __restrict__ npid, int* __restrict__ nmol,
-                          int* __restrict__ slot_of) {
+                          int* __restrict__ slot_of, int n_cells, DStatus* status) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_total) return;
-  int d = cstart[ucell[s]] + urank[s];
+  const int c = ucell[s];
+  if (c < 0 || c >= n_cells) {
+    atomicOr(&status->integrity, 1);
+    return;
+  }
+  int d = cstart[c] + urank[s];
+  if (d < 0 || d >= n_total) {
+    atomicOr(&status->integrity, 1);
+    return;
+  }
   int p = pid[s];
   npos[d] = upos[s];
   ntype[d] = type[s];
@@ -919,6 +1078,15 @@ __global__ void k_cells_out(int n_total, const DRep* __restrict__ rep,
   const DRep& R = rep[rep_of[p]];
   double2 x = pos[s];
   out[p] = R.cbase + cell_coord(x.y, R.cl[1], R.ly) * R.lx + cell_coord(x.x, R.cl[0], R.lx);
+}
+
+// Fills the LDS of every workgroup slot with a word pattern (test hook: NaN
+// patterns make later reads of never-written LDS visible).
+__global__ __launch_bounds__(256) void k_fill_lds(unsigned pattern, int words) {
+  extern __shared__ unsigned lds_words[];
+  for (int q = threadIdx.x; q < words; q += blockDim.x) lds_words[q] = pattern;
+  __syncthreads();
+  if (threadIdx.x == 0 && lds_words[0] != pattern) lds_words[1] = 0;  // keep the stores
 }
 
 __global__ void k_load(int n_total, const double2* __restrict__ x, const int* __restrict__ t,
@@ -994,6 +1162,13 @@ struct rds_ctx {
   hipGraphExec_t graph_exec = nullptr;
   int graph_len = 0;
   int graph_parity = 0;
+  // kernel variant
+  int rows = 1;         // cell rows per tile (1, 2 or 4)
+  bool fast = false;    // all models: <= 2 types, one LJ term per pair, same even n
+  int pw = 0;           // n / 2 of that LJ exponent
+  int cap = 384;        // staged particles per LDS round
+  bool lists = false;   // two-pass force evaluation (partner lists in LDS)
+  size_t smem_bytes = 0;
 };
 
 namespace {
@@ -1024,21 +1199,42 @@ StepArgs make_args(rds_ctx* c, const Buf& cur) {
   a.inj_count = c->inj_count;
   a.rng = c->rng;
   a.prob_mode = c->prob_mode;
+  a.cap = c->cap;
+  a.rep_of = c->d_rep_of;
   a.include_rf = c->include_rf;
   a.status = c->d_status;
   return a;
 }
 
-template <int MODE>
-void launch_force(rds_ctx* c, const StepArgs& a) {
-  dim3 g((unsigned)c->n_tiles), b(FBLOCK);
-  if (c->precision == RDS_FP64) {
-    if (c->has_mol) hipLaunchKernelGGL((k_force<double, true, MODE>), g, b, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_force<double, false, MODE>), g, b, 0, c->stream, a);
-  } else {
-    if (c->has_mol) hipLaunchKernelGGL((k_force<float, true, MODE>), g, b, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_force<float, false, MODE>), g, b, 0, c->stream, a);
+template <typename Real, bool MOL, int KM, bool LISTS>
+void launch_force_r(rds_ctx* c, const StepArgs& a, int mode) {
+  dim3 g((unsigned)c->n_tiles);
+  const size_t sm = c->smem_bytes;
+  switch (c->rows) {
+    case 4: hipLaunchKernelGGL((k_force<Real, MOL, KM, 4, LISTS>), g, dim3(256), sm, c->stream, a, mode, c->pw); break;
+    case 2: hipLaunchKernelGGL((k_force<Real, MOL, KM, 2, LISTS>), g, dim3(128), sm, c->stream, a, mode, c->pw); break;
+    default: hipLaunchKernelGGL((k_force<Real, MOL, KM, 1, LISTS>), g, dim3(64), sm, c->stream, a, mode, c->pw); break;
   }
+}
+
+template <typename Real, bool MOL>
+void launch_force_m(rds_ctx* c, const StepArgs& a, int mode) {
+  if (!c->fast) launch_force_r<Real, MOL, 0, false>(c, a, mode);
+  else if (c->pw == 3) {
+    if (c->lists) launch_force_r<Real, MOL, 2, true>(c, a, mode);
+    else launch_force_r<Real, MOL, 2, false>(c, a, mode);
+  } else launch_force_r<Real, MOL, 1, false>(c, a, mode);
+}
+
+template <typename Real>
+void launch_force_p(rds_ctx* c, const StepArgs& a, int mode) {
+  if (c->has_mol) launch_force_m<Real, true>(c, a, mode);
+  else launch_force_m<Real, false>(c, a, mode);
+}
+
+void launch_force(rds_ctx* c, const StepArgs& a, int mode) {
+  if (c->precision == RDS_FP64) launch_force_p<double>(c, a, mode);
+  else launch_force_p<float>(c, a, mode);
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -1077,25 +1273,27 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
   }
   StepArgs a = make_args(c, cur);
   mark(KT_FORCE, true);
-  launch_force<MODE_STEP>(c, a);
+  launch_force(c, a, MODE_STEP);
   mark(KT_FORCE, false);
   mark(KT_SCAN, true);
   hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
-                     c->cstart, c->d_step, 1);
+                     c->cstart, c->d_step, 1, c->d_status);
   mark(KT_SCAN, false);
   mark(KT_SCATTER, true);
   hipLaunchKernelGGL(k_scatter, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
                      (int)c->n_total, c->upos, c->ucell, c->urank, c->cstart, cur.type, cur.pid,
                      cur.mol, nxt.pos, c->any_rx ? nxt.type0 : nxt.type, nxt.pid, nxt.mol,
-                     c->any_bonds ? c->slot_of : nullptr);
+                     c->any_bonds ? c->slot_of : nullptr, (int)c->n_cells, c->d_status);
   mark(KT_SCATTER, false);
   if (c->any_rx) {
     StepArgs r = make_args(c, nxt);
     r.type = nxt.type0;
     r.type_out = nxt.type;
     mark(KT_REACT, true);
-    if (c->has_mol) hipLaunchKernelGGL((k_react<true>), dim3((unsigned)c->n_tiles), dim3(FBLOCK), 0, c->stream, r);
-    else hipLaunchKernelGGL((k_react<false>), dim3((unsigned)c->n_tiles), dim3(FBLOCK), 0, c->stream, r);
+    if (c->has_mol)
+      hipLaunchKernelGGL((k_react<true>), dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream, r);
+    else
+      hipLaunchKernelGGL((k_react<false>), dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream, r);
     mark(KT_REACT, false);
   }
   c->parity ^= 1;
@@ -1200,6 +1398,8 @@ int fill_model(const rds_model& m, DModel& d, double* rcut) {
     }
   // terms whose pair is not sorted / out of range never match (pyrdsim.pyx:1999-2007)
   d.nterms = k;
+  d.rc2max = 0.0;
+  for (int q = 0; q < k; ++q) d.rc2max = std::max(d.rc2max, d.terms[q].rc2);
   for (int q = 0; q < m.n_walls; ++q) {
     const rds_wall_term& w = m.walls[q];
     if (w.direction < 0 || w.direction > 1) return fail(RDS_E_ARG, "wall direction must be 0 or 1");
@@ -1220,6 +1420,8 @@ int fill_model(const rds_model& m, DModel& d, double* rcut) {
     o.inh2 = r.inhibitor2;
     o.r0 = r.r0;
     o.rate_decrease = r.rate_decrease;
+    o.r0sq_lo = r.r0 * r.r0 * (1.0 - 1e-14);
+    o.r0sq_hi = r.r0 * r.r0 * (1.0 + 1e-14);
     if (r.reactant < 0 || r.reactant >= nt || r.product < 0 || r.product >= nt)
       return fail(RDS_E_ARG, "reaction %d: reactant/product outside the %d particle types", q, nt);
     if (r.kind == RDS_RX_CATALYZED || r.kind == RDS_RX_INHIBITED) {
@@ -1282,7 +1484,12 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   c->graph_steps = cfg->graph_steps;
   c->n_rep = cfg->n_replicas;
   c->n_models = cfg->n_models;
-  const int spc = cfg->cells_per_cutoff > 0 ? cfg->cells_per_cutoff : 1;
+  c->rows = cfg->tile_rows > 0 ? cfg->tile_rows : 1;
+  c->lists = (cfg->cells_per_cutoff < 0);
+  if (c->rows != 1 && c->rows != 2 && c->rows != 4)
+    return bail(fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4"));
+  if (cfg->n_replicas >= (1 << 24)) return bail(fail(RDS_E_ARG, "too many replicas"));
+  const int spc = cfg->cells_per_cutoff != 0 ? std::abs(cfg->cells_per_cutoff) : 1;
   if (spc != 1) return bail(fail(RDS_E_ARG, "only cells_per_cutoff = 1 (3x3 stencil) is built"));
   const double target = cfg->tile_particles > 0 ? cfg->tile_particles : 56;
 
@@ -1293,9 +1500,26 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     if (rc) return bail(rc);
     if (cfg->models[m].n_reactions > 0) c->any_rx = true;
   }
+  // register-resident LJ path: every model <= 2 types, <= 1 term per type
+  // pair, all LJ with one common even exponent
+  c->fast = true;
+  c->pw = 0;
+  for (int m = 0; m < cfg->n_models && c->fast; ++m) {
+    const DModel& d = c->h_models[m];
+    if (d.nt > 2) c->fast = false;
+    for (int q = 0; q < d.nt * d.nt && c->fast; ++q)
+      if (d.pcount[q] > 1) c->fast = false;
+    for (int q = 0; q < d.nterms && c->fast; ++q) {
+      const DTerm& t = d.terms[q];
+      if (t.kind != RDS_LJ || t.odd || (c->pw && t.pw != c->pw)) c->fast = false;
+      else c->pw = t.pw;
+    }
+  }
+  if (c->pw == 0) c->pw = 1;
   if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(RDS_E_HIP, "no HIP device"));
   c->h_rep.resize(c->n_rep);
   int64_t pbase = 0, cbase = 0;
+  double staged_worst = 0.0;
   for (int r = 0; r < c->n_rep; ++r) {
     const rds_replica& s = cfg->replicas[r];
     DRep& d = c->h_rep[r];
@@ -1348,11 +1572,13 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     W = std::max(1, std::min(W, std::min(MAXW, d.lx)));
     int nstrip = (d.lx + W - 1) / W;
     int Wq = d.lx / nstrip, Wr = d.lx % nstrip;
-    for (int y = 0; y < d.ly; ++y) {
+    staged_worst = std::max(staged_worst, occ * (c->rows + 2) * (Wq + (Wr ? 1 : 0) + 2));
+    for (int y = 0; y < d.ly; y += c->rows) {
+      int nr = std::min(c->rows, d.ly - y);
       int x0 = 0;
       for (int k = 0; k < nstrip; ++k) {
         int wk = Wq + (k < Wr ? 1 : 0);
-        c->h_tiles.push_back(make_int4(r, y, x0, wk));
+        c->h_tiles.push_back(make_int4(r | ((nr - 1) << 24), y, x0, wk));
         x0 += wk;
       }
     }
@@ -1364,6 +1590,17 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   c->n_total = pbase;
   c->n_cells = cbase;
   c->n_tiles = (int64_t)c->h_tiles.size();
+  // LDS round capacity: 1.5x the mean staged count of the densest tile shape,
+  // at least 256, at most what 64 KB of dynamic LDS holds; denser tiles stage
+  // in several rounds
+  {
+    const int per = 16 + 4 + (c->has_mol ? 4 : 0);
+    int cap = (int)std::ceil(1.5 * staged_worst / 64.0) * 64;
+    cap = std::max(cap, 256);
+    cap = std::min(cap, (65536 - 256) / per / 64 * 64);
+    c->cap = cap;
+    c->smem_bytes = (size_t)cap * per;
+  }
 
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t N = (size_t)c->n_total, C = (size_t)c->n_cells;
@@ -1459,18 +1696,18 @@ int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32
                      c->has_mol ? a.mol : nullptr);
   hipLaunchKernelGGL(k_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep, c->d_rep_of,
                      a.pos, c->ucell, c->urank, c->count);
-  hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
-                     c->cstart, c->d_step, 0);
-  hipLaunchKernelGGL(k_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos, c->ucell,
-                     c->urank, c->cstart, a.type, a.pid, a.mol, b.pos, b.type, b.pid, b.mol,
-                     c->slot_of);
-  c->parity = 1;
-  long long s0 = step0;
-  HIP_TRY(hipMemcpyAsync(c->d_step, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream));
   DStatus st;
   memset(&st, 0, sizeof st);
   st.err_key = ~0ull;
   HIP_TRY(hipMemcpyAsync(c->d_status, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
+                     c->cstart, c->d_step, 0, c->d_status);
+  hipLaunchKernelGGL(k_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos, c->ucell,
+                     c->urank, c->cstart, a.type, a.pid, a.mol, b.pos, b.type, b.pid, b.mol,
+                     c->slot_of, (int)c->n_cells, c->d_status);
+  c->parity = 1;
+  long long s0 = step0;
+  HIP_TRY(hipMemcpyAsync(c->d_step, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipGetLastError());
   c->loaded = true;
@@ -1540,6 +1777,7 @@ int rds_sync(rds_ctx* c, rds_status* out) {
   s.bounds_error = st.bounds;
   s.stream_error = st.stream;
   s.reaction_events = (int64_t)st.events;
+  s.integrity = st.integrity;
   if (st.bounds) {
     s.error_step = (int64_t)(st.err_key >> 32);
     unsigned lo = (unsigned)(st.err_key & 0xffffffffu);
@@ -1548,11 +1786,13 @@ int rds_sync(rds_ctx* c, rds_status* out) {
     s.error_value = st.err_val;
   }
   if (out) *out = s;
+  if (st.integrity)
+    return fail(RDS_E_STATE, "internal cell-list invariant violated (flags %d)", st.integrity);
   if (st.bounds)
     return fail(RDS_E_BOUNDS,
                 "x[%d,%d] = %.17g out of bounds. Maybe the timestep is too large for the given "
-                "the interaction parameters?",
-                s.error_particle, s.error_axis, s.error_value);
+                "the interaction parameters? (step %lld)",
+                s.error_particle, s.error_axis, s.error_value, (long long)s.error_step);
   if (st.stream) return fail(RDS_E_STREAM, "injected random stream exhausted");
   return RDS_OK;
 }
@@ -1567,7 +1807,7 @@ int rds_compute_forces(rds_ctx* c, double* forces) {
                        c->boff, c->bpart, c->br0, c->bu0, c->fbond);
   StepArgs a = make_args(c, cur);
   a.fout = (double2*)forces;
-  launch_force<MODE_FORCE>(c, a);
+  launch_force(c, a, MODE_FORCE);
   HIP_TRY(hipGetLastError());
   return RDS_OK;
 }
@@ -1610,7 +1850,8 @@ int rds_neighbor_counts(rds_ctx* c, double r, int32_t* counts) {
   StepArgs a = make_args(c, cur);
   a.iout = counts;
   a.rprobe2 = r * r;
-  launch_force<MODE_COUNT>(c, a);
+  if (c->has_mol) hipLaunchKernelGGL((k_count<true>), dim3((unsigned)c->n_tiles), dim3(64), c->smem_bytes, c->stream, a);
+  else hipLaunchKernelGGL((k_count<false>), dim3((unsigned)c->n_tiles), dim3(64), c->smem_bytes, c->stream, a);
   HIP_TRY(hipGetLastError());
   return RDS_OK;
 }
@@ -1639,6 +1880,15 @@ int rds_time_kernels(rds_ctx* c, int64_t n_steps, double* avg_ms, int64_t* count
     if (counts) counts[k] = cnt[k];
   }
   HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+int rds_debug_fill_lds(rds_ctx* c, uint32_t pattern) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  const int words = 60 * 1024 / 4;  // 60 KB per workgroup, many workgroups per CU
+  hipLaunchKernelGGL(k_fill_lds, dim3(4096), dim3(256), words * 4, c->stream, pattern, words);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return RDS_OK;
 }
 

@@ -129,7 +129,7 @@ class Engine:
 
     def __init__(self, models, replicas, precision='mixed', rng='philox',
                  reaction_prob='linear', bonds=None, has_molecules=False,
-                 tile_particles=0, graph_steps=0, device='cuda'):
+                 tile_particles=0, graph_steps=0, tile_rows=1, lists=False, device='cuda'):
         self.lib = L.load()
         if not torch.cuda.is_available():
             raise RuntimeError('the engine needs a ROCm GPU; none is visible')
@@ -151,8 +151,9 @@ class Engine:
         cfg.reaction_prob = PROBS[reaction_prob]
         cfg.has_molecules = 1 if has_molecules else 0
         cfg.tile_particles = int(tile_particles)
-        cfg.cells_per_cutoff = 1
+        cfg.cells_per_cutoff = -1 if lists else 1
         cfg.graph_steps = int(graph_steps)
+        cfg.tile_rows = int(tile_rows)
         self._bond_arrays = None
         if bonds is not None and len(bonds[0]) > 0:
             bi = np.ascontiguousarray(bonds[0], dtype=np.int32)
@@ -289,6 +290,10 @@ class Engine:
                                           cnt.ctypes.data))
         self.step += int(n_steps)
         return {k: (float(avg[i]), int(cnt[i])) for i, k in enumerate(L.KERNELS)}
+
+    def debug_fill_lds(self, pattern=0x7fc00001):
+        """Test hook: fill LDS with a (NaN) pattern before the next launch."""
+        L.check(self.lib.rds_debug_fill_lds(self.ctx, int(pattern) & 0xffffffff))
 
     def sizes(self):
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()

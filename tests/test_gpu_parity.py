@@ -193,3 +193,62 @@ def test_out_of_bounds_raises(gpu):
     eng.run(1)
     with pytest.raises(RuntimeError, match='out of bounds'):
         eng.sync()
+
+
+@pytest.mark.parametrize('rows,per_tile', [(1, 8), (2, 20), (4, 56), (1, 400), (4, 400)])
+@pytest.mark.parametrize('name', ['kalj', 'wall', 'patchy'])
+def test_tile_shapes(gpu, name, rows, per_tile):
+    """Every tile shape gives the same forces and the same injected-noise
+    trajectory: small strips, multi-row tiles, home rows of more than one wave
+    batch and staged regions larger than one LDS round (per_tile = 400)."""
+    s = small(name)
+    if name == 'wall':
+        s['types'][::3] = 1
+    if name == 'patchy':
+        s['types'][2::4] = 2
+    ref = engine_for(s, precision='fp64')
+    eng = engine_for(s, precision='fp64', tile_rows=rows, tile_particles=per_tile)
+    f0, f1 = ref.forces(), eng.forces()
+    scale = np.abs(f0).max() + 1.0
+    assert np.abs(f1 - f0).max() < 1e-12 * scale
+    steps = 12
+    normals, u1, u2 = _injected(len(s['x']), steps, 5)
+    u1 *= 1e-3
+    outs = []
+    for e in (ref, eng):
+        e.close()
+    for kw in (dict(), dict(tile_rows=rows, tile_particles=per_tile)):
+        e = engine_for(s, precision='fp64', rng='injected', **kw)
+        e.set_injected(normals, u1, u2, first_step=1)
+        e.run(steps)
+        e.sync()
+        outs.append(e.state())
+    d = outs[0][0] - outs[1][0]
+    d = d - s['L'] * np.round(d / s['L']) * (s['periodic'] > 0)
+    assert np.abs(d).max() < 1e-12
+    assert np.array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize('rows,per_tile', [(1, 56), (2, 56), (4, 112)])
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_no_reads_of_unstaged_lds(gpu, rows, per_tile, precision):
+    """Regression: with the LDS of the GPU filled with NaN bits before the
+    launch, forces and one step stay finite and equal to the oracle's -- no
+    kernel reads LDS it did not stage (a masked extra candidate once did)."""
+    s = small('kalj', n=4000, seed=11)
+    eng = engine_for(s, precision=precision, tile_rows=rows, tile_particles=per_tile)
+    o = oracle_for(s)
+    o.init()
+    _, _, a = o.state()
+    for _ in range(3):
+        eng.debug_fill_lds()
+        f = eng.forces()
+        assert np.isfinite(f).all()
+        err = np.abs(f - a).max(axis=1)
+        tol = (1e-10 if precision == 'fp64' else 1e-5) * (force_scale(s) + np.abs(a).max(axis=1))
+        assert ((err <= tol) | near_cutoff(s)).all()
+    eng.debug_fill_lds()
+    eng.run(1)
+    eng.sync()
+    x, _ = eng.state()
+    assert np.isfinite(x).all()
