@@ -233,7 +233,10 @@ struct Oracle {
     double r = (double)s.ratio;
     for (int k = 0; k < d; ++k) {
       int l = 1;
-      while (s.L[k] / (double)(l + 1) >= s.rs / r) ++l;
+      // With rs = 0 (no interaction range and no rs given) the reference's
+      // loop never ends; one cell is the only meaningful grid then.
+      if (s.rs > 0.0)
+        while (s.L[k] / (double)(l + 1) >= s.rs / r) ++l;
       cl[k] = s.L[k] / (double)l;
       ncell[k] = l;
       pbc_nc[k] = l - s.ratio;
@@ -306,6 +309,12 @@ struct Oracle {
 
   // pyx:1448-1587 (faithful) or every pair within rs_exact (exact)
   void create_neighbor_list() {
+    if (s.nl_mode == NL_EXACT && !(rs_exact > 0.0) && bonds.empty()) {
+      // no pair interaction, reaction radius or bond: nothing to list
+      nl.assign(n, {});
+      calc_distances();
+      return;
+    }
     if (s.nl_mode == NL_EXACT) {
       nl.assign(n, {});
       double dx[3], dist;

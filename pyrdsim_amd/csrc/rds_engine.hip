@@ -159,6 +159,7 @@ struct StepArgs {
   // staged particles per LDS round (dynamic shared memory)
   int cap;
   const int* rep_of;  // replica of each pid
+  const int4* sblocks;  // (replica, first slot, end slot) blocks of <= 256 slots
 };
 
 // ---------------------------------------------------------------------------
@@ -802,17 +803,43 @@ __global__ __launch_bounds__(64) void k_count(const StepArgs A) {
 // 3x3 cells (reading neighbours straight from the sorted arrays).
 // ---------------------------------------------------------------------------
 
+// rate of reaction q for a particle of type ti with cnt neighbours, accumulated
+// as the reference does (repeated addition, pyrdsim.pyx:2109-2206); `total`
+// receives the same increments in the same order.
+__device__ __forceinline__ double rx_rate(const DRx& rx, double rate, int ti, int cnt,
+                                          double& total) {
+  if (rx.reactant != ti) return 0.0;
+  if (rx.kind == RDS_RX_LINEAR) {
+    total += rate;
+    return rate;
+  }
+  if (rx.kind == RDS_RX_CATALYZED) {
+    double acc = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+      acc += rate;
+      total += rate;
+    }
+    return acc;
+  }
+  double acc = rate;
+  for (int k = 0; k < cnt; ++k) acc -= rx.rate_decrease;
+  if (acc < 0) acc = 0.0;
+  total += acc;
+  return acc;
+}
+
 template <bool MOL>
 __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= A.n_total) return;
-  const int p = A.pid[s];
-  const DRep& Rp = A.rep[A.rep_of[p]];
+  const int4 B = A.sblocks[blockIdx.x];  // (replica, first slot, end slot)
+  const int s = B.y + (int)threadIdx.x;
+  if (s >= B.z) return;
+  const DRep& Rp = A.rep[B.x];
   const DModel& M = A.models[Rp.model];
   const int nrx = M.nrx;
   const int ti = A.type[s];
   int tnew = ti;
   if (nrx > 0) {
+    const int p = A.pid[s];
     const long long step = *A.step;  // already advanced: the reference's l+1
     bool need = false;
     for (int q = 0; q < nrx; ++q)
@@ -861,34 +888,12 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
         }
       }
     }
-    // rates in reaction order (pyrdsim.pyx:2103-2206)
-    double rates[MAXRX];
+    // total rate in reaction order (pyrdsim.pyx:2103-2206)
     double total_rate = 0.0;
-#pragma unroll
-    for (int q = 0; q < MAXRX; ++q) rates[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < MAXRX; ++q) {
       if (q >= nrx) break;
-      const DRx& rx = M.rx[q];
-      const double rate = Rp.rate[q];
-      if (rx.reactant != ti) continue;
-      if (rx.kind == RDS_RX_LINEAR) {
-        rates[q] = rate;
-        total_rate += rate;
-      } else if (rx.kind == RDS_RX_CATALYZED) {
-        double acc = 0.0;
-        for (int k = 0; k < cnt[q]; ++k) {
-          acc += rate;
-          total_rate += rate;
-        }
-        rates[q] = acc;
-      } else {
-        double acc = rate;
-        for (int k = 0; k < cnt[q]; ++k) acc -= rx.rate_decrease;
-        if (acc < 0) acc = 0.0;
-        rates[q] = acc;
-        total_rate += acc;
-      }
+      rx_rate(M.rx[q], Rp.rate[q], ti, cnt[q], total_rate);
     }
     if (total_rate > 0.0) {
       double u1, u2;
@@ -908,12 +913,14 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
       double prob = (A.prob_mode == RDS_PROB_EXP) ? -expm1(-Rp.dt * total_rate)
                                                   : Rp.dt * total_rate;
       if (prob > u1) {
-        double sacc = 0.0;
+        // choice: first reaction whose cumulative share reaches u2
+        // (pyrdsim.pyx:2215-2222); the rates are recomputed bit-identically
+        double sacc = 0.0, dummy = 0.0;
         int chosen = -1;
 #pragma unroll
         for (int q = 0; q < MAXRX; ++q) {
           if (q >= nrx || chosen >= 0) break;
-          sacc += rates[q] / total_rate;
+          sacc += rx_rate(M.rx[q], Rp.rate[q], ti, cnt[q], dummy) / total_rate;
           if (sacc >= u2) chosen = q;
         }
         if (chosen >= 0) {
@@ -1132,6 +1139,8 @@ struct rds_ctx {
   DModel* d_models = nullptr;
   int4* d_tiles = nullptr;
   int* d_rep_of = nullptr;  // replica of each pid
+  int4* d_sblocks = nullptr;  // per-replica slot blocks of the reaction kernel
+  int64_t n_sblocks = 0;
   Buf buf[2];
   int parity = 0;
   double2* upos = nullptr;
@@ -1201,6 +1210,7 @@ StepArgs make_args(rds_ctx* c, const Buf& cur) {
   a.prob_mode = c->prob_mode;
   a.cap = c->cap;
   a.rep_of = c->d_rep_of;
+  a.sblocks = c->d_sblocks;
   a.include_rf = c->include_rf;
   a.status = c->d_status;
   return a;
@@ -1291,9 +1301,9 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
     r.type_out = nxt.type;
     mark(KT_REACT, true);
     if (c->has_mol)
-      hipLaunchKernelGGL((k_react<true>), dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream, r);
+      hipLaunchKernelGGL((k_react<true>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
     else
-      hipLaunchKernelGGL((k_react<false>), dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream, r);
+      hipLaunchKernelGGL((k_react<false>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
     mark(KT_REACT, false);
   }
   c->parity ^= 1;
@@ -1447,7 +1457,7 @@ int rds_ctx_destroy(rds_ctx* c) {
   if (!c) return RDS_OK;
   free_graph(c);
   if (c->stream) hipStreamSynchronize(c->stream);
-  void* ptrs[] = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->upos, c->ucell, c->urank,
+  void* ptrs[] = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->d_sblocks, c->upos, c->ucell, c->urank,
                   c->count, c->cstart, c->d_step, c->d_status, c->boff, c->bpart, c->br0,
                   c->bu0, c->fbond, c->slot_of};
   for (void* p : ptrs)
@@ -1631,9 +1641,17 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   HIP_TRY(hipMemcpy(c->d_tiles, c->h_tiles.data(), sizeof(int4) * c->n_tiles, hipMemcpyHostToDevice));
   {
     std::vector<int> rep_of(N);
-    for (int r = 0; r < c->n_rep; ++r)
-      for (int i = 0; i < c->h_rep[r].n; ++i) rep_of[c->h_rep[r].pbase + i] = r;
+    std::vector<int4> sb;
+    for (int r = 0; r < c->n_rep; ++r) {
+      const DRep& d = c->h_rep[r];
+      for (int i = 0; i < d.n; ++i) rep_of[d.pbase + i] = r;
+      for (int b = 0; b < d.n; b += 256)
+        sb.push_back(make_int4(r, d.pbase + b, d.pbase + std::min(b + 256, d.n), 0));
+    }
     HIP_TRY(hipMemcpy(c->d_rep_of, rep_of.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    c->n_sblocks = (int64_t)sb.size();
+    if ((rc = dalloc(&c->d_sblocks, sb.size())) != RDS_OK) return bail(rc);
+    HIP_TRY(hipMemcpy(c->d_sblocks, sb.data(), sizeof(int4) * sb.size(), hipMemcpyHostToDevice));
   }
   HIP_TRY(hipMemset(c->count, 0, sizeof(int) * C));
   {

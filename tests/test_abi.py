@@ -1,0 +1,77 @@
+"""The C ABI library (CPU only: no GPU call is made).
+
+Checks that librds_hip.so loads, reports its ABI version, exports every entry
+point include/rds.h declares, and that the ctypes layouts match the header's
+struct definitions field by field.
+"""
+
+import ctypes as C
+import os
+import re
+
+from pyrdsim_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'rds.h')
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r'^(?:int|const char\*)\s+(rds_\w+)\s*\(', text, re.M)))
+
+
+def test_library_loads_and_reports_version():
+    lib = _lib.load()
+    assert lib.rds_abi_version() == _lib.ABI_VERSION
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, name
+    assert set(_lib.SIGNATURES) == set(names)
+
+
+def _header_struct_fields(name):
+    text = open(HEADER).read()
+    end = re.search(r'\}\s*' + name + ';', text).start()
+    start = text.rindex('typedef struct {', 0, end) + len('typedef struct {')
+    body = re.sub(r'/\*.*?\*/', '', text[start:end], flags=re.S)
+    fields = []
+    for decl in body.split(';'):
+        decl = decl.strip()
+        if not decl:
+            continue
+        parts = [w for w in decl.replace('*', ' ').split() if w != 'const']
+        for f in ' '.join(parts[1:]).split(','):
+            f = f.strip()
+            if f in ('int32_t', 'double', 'uint64_t', 'int64_t'):
+                continue
+            fields.append(re.sub(r'\[.*\]', '', f).strip())
+    return [f for f in fields if f and f not in ('const', 'rds_model', 'rds_replica',
+                                                  'rds_pair_term', 'rds_wall_term',
+                                                  'rds_reaction_def')]
+
+
+def test_struct_layouts_follow_the_header():
+    pairs = [('rds_pair_term', _lib.PairTerm), ('rds_wall_term', _lib.WallTerm),
+             ('rds_reaction_def', _lib.ReactionDef), ('rds_model', _lib.Model),
+             ('rds_replica', _lib.Replica), ('rds_config', _lib.Config),
+             ('rds_status', _lib.Status)]
+    for cname, py in pairs:
+        want = _header_struct_fields(cname)
+        got = [f[0] for f in py._fields_]
+        assert got == want, (cname, got, want)
+    assert C.sizeof(_lib.PairTerm) == 56 and C.sizeof(_lib.WallTerm) == 32
+    assert C.sizeof(_lib.ReactionDef) == 48
+    assert C.sizeof(_lib.Model) == 16 + 32 * 56 + 8 * 32 + 8 * 48
+
+
+def test_errors_without_context():
+    lib = _lib.load()
+    assert lib.rds_run(None, 1) == _lib.E_ARG
+    assert b'null' in lib.rds_last_error()
+    assert lib.rds_sync(None, None) == _lib.E_ARG
