@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from pyrdsim_amd import workloads  # noqa: E402
+from pyrdsim_amd.autotune import autotune  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FORCE_BYTES_PER_PARTICLE = 48   # read pos 16 + type 4 + pid 4; write pos 16 + cell 4 + rank 4
@@ -102,6 +103,39 @@ def time_engine(eng, steps, warmup):
     return time.perf_counter() - t0
 
 
+def accuracy_leg(plan, states, models, reps, x_mixed, t_mixed, steps, n_check):
+    """RDF and activated fraction of the batched FP32-force run against the
+    FP64 path: a stratified subset of replicas is re-run alone in FP64 for the
+    same number of steps with the same Philox streams, from the same states."""
+    from pyrdsim_amd import analysis
+    from pyrdsim_amd.engine import Engine
+    off = np.concatenate([[0], np.cumsum([p['n'] for p in plan])])
+    picks = []
+    for kind in ('kalj', 'wall'):
+        idx = sorted((r for r, p in enumerate(plan) if p['kind'] == kind),
+                     key=lambda r: plan[r]['n'])
+        m = max(1, n_check // 2)
+        picks += [idx[int((i + 0.5) * len(idx) / m)] for i in range(m)]
+    rdf_err, act_err = [], []
+    for r in picks:
+        p = plan[r]
+        e = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64')
+        e.set_state(states[r][0], states[r][1])
+        e.run(steps)
+        x64, t64 = e.state()
+        e.close()
+        L = [reps[r].L[0], reps[r].L[1]]
+        per = [reps[r].periodic[0], reps[r].periodic[1]]
+        xm = x_mixed[off[r]:off[r + 1]]
+        tm = t_mixed[off[r]:off[r + 1]]
+        _, g64 = analysis.rdf(x64, L, per)
+        _, gm = analysis.rdf(xm, L, per)
+        rdf_err.append(analysis.rdf_error(gm, g64))
+        act_err.append(abs(analysis.activated_fraction(tm) - analysis.activated_fraction(t64)))
+    return dict(replicas=len(picks), steps=int(steps), reference='fp64 single replica, same streams',
+                rdf_l1_rel_max=float(max(rdf_err)), activated_fraction_abs_max=float(max(act_err)))
+
+
 def cpu_baseline(plan, states, budget_s):
     """The oracle, reference-faithful (Verlet list, rs = 2 rc, ratio 2, refresh
     every 10 steps, FP64, one core), on the smallest replica of each model."""
@@ -153,6 +187,12 @@ def main():
     ap.add_argument('--lists', action='store_true')
     ap.add_argument('--ic-cache', default=None, help='.npz file of initial states (read or written)')
     ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--no-autotune', dest='autotune', action='store_false')
+    ap.add_argument('--no-accuracy', action='store_true')
+    ap.add_argument('--accuracy-replicas', type=int, default=8)
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='run one elementwise kernel of known traffic (256 MiB read + 256 MiB '
+                         'written) first, to scale HBM counters in a profile of this command')
     ap.add_argument('--no-cpu', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
@@ -167,6 +207,13 @@ def main():
 
     import torch
     from pyrdsim_amd.engine import Engine
+    if args.copy_calibration:
+        a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device='cuda')
+        torch.cuda.synchronize()
+        b = a * 2.0          # reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        del a, b
+        print('# copy calibration: 268435456 B read + 268435456 B written', file=sys.stderr)
     models, reps = engine_pieces(plan)
     n_tot = sum(p['n'] for p in plan)
     x0 = np.concatenate([s[0] for s in states])
@@ -176,6 +223,11 @@ def main():
                  tile_particles=args.tile_particles, tile_rows=args.tile_rows,
                  lists=args.lists)
     eng.set_state(x0, t0s)
+    tuned = None
+    if args.autotune:
+        best, times = autotune(eng)
+        tuned = dict(best=list(best), force_ms={'%dx%d' % k: round(v, 5) for k, v in times.items()})
+        print('# autotune: rows x particles = %s' % (best,), file=sys.stderr, flush=True)
     secs = time_engine(eng, args.steps, args.warmup)
     value = n_tot * args.steps / secs
     print('# batched: %.3e particle-steps/s (%.3f ms/step)' % (value, 1e3 * secs / args.steps),
@@ -192,8 +244,17 @@ def main():
                     kernel='k_force (pair forces + Euler-Maruyama + binning)',
                     algorithmic_bytes=int(force_bytes),
                     avg_ms={k: round(v[0], 5) for k, v in kt.items() if v[1]})
+    x_mixed, t_mixed = eng.state()
+    steps_done = eng.step
+    tiling = eng.tiling()
     del eng
     torch.cuda.synchronize()
+
+    accuracy = None
+    if not args.no_accuracy:
+        accuracy = accuracy_leg(plan, states, models, reps, x_mixed, t_mixed, steps_done,
+                                args.accuracy_replicas)
+        print('# accuracy: %s' % accuracy, file=sys.stderr, flush=True)
 
     plain = None
     if not args.no_plain:
@@ -203,17 +264,26 @@ def main():
         pick = [int(order[int((i + 0.5) * len(order) / k)]) for i in range(k)]
         work = 0.0
         tsum = 0.0
+        plain_tiling = (args.tile_rows, args.tile_particles)
+        if args.autotune:
+            # tuned once, on the median-size replica of the subset
+            r = pick[len(pick) // 2]
+            e1 = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64')
+            e1.set_state(states[r][0], states[r][1])
+            plain_tiling, _ = autotune(e1, steps=8)
+            e1.close()
         for r in pick:
             e1 = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64',
-                        graph_steps=args.graph_steps, tile_particles=args.tile_particles,
-                        tile_rows=args.tile_rows)
+                        graph_steps=args.graph_steps, tile_rows=plain_tiling[0],
+                        tile_particles=plain_tiling[1])
             e1.set_state(states[r][0], states[r][1])
             dt = time_engine(e1, args.plain_steps, 20)
             work += plan[r]['n'] * args.plain_steps
             tsum += dt
             e1.close()
         plain = dict(value=work / tsum, unit='particle-steps/s', replicas=k,
-                     steps=args.plain_steps, precision='fp64', graph_steps=args.graph_steps)
+                     steps=args.plain_steps, precision='fp64', graph_steps=args.graph_steps,
+                     tiling=list(plain_tiling))
         print('# plain: %.3e particle-steps/s' % plain['value'], file=sys.stderr, flush=True)
 
     cpu = None
@@ -240,11 +310,12 @@ def main():
                         'wall+reactions rho=0.4 dt=1e-4), N log-uniform 2048..32768, '
                         'sum N=%d' % (len(plan), n_tot),
             'precision': 'mixed', 'graph_steps': args.graph_steps,
-            'tile_rows': args.tile_rows, 'tile_particles': args.tile_particles,
+            'tiling': list(tiling[:2]), 'lds_cap': tiling[2], 'autotune': tuned,
             'cells': int(n_cells), 'tiles': int(n_tiles)},
         'roofline': roofline,
         'plain': plain,
         'batched_vs_plain': (value / plain['value']) if plain else None,
+        'accuracy': accuracy,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))

@@ -206,6 +206,13 @@ int rds_time_kernels(rds_ctx* ctx, int64_t n_steps, double* avg_ms, int64_t* cou
  * LDS it never wrote would see that pattern (NaN bits make it visible). */
 int rds_debug_fill_lds(rds_ctx* ctx, uint32_t pattern);
 
+/* Re-tiles a context (kernel work decomposition only; the state is kept):
+ * tile_rows cell rows per tile (1, 2, 4) and about tile_particles particles
+ * per tile row strip.  No counterpart in the reference; used by the
+ * autotuner.  rds_get_tiling reports the current choice and LDS capacity. */
+int rds_set_tiling(rds_ctx* ctx, int32_t tile_rows, int32_t tile_particles);
+int rds_get_tiling(rds_ctx* ctx, int32_t* tile_rows, int32_t* tile_particles, int32_t* lds_cap);
+
 /* Sizes: n_total particles, n_cells, n_tiles. */
 int rds_sizes(rds_ctx* ctx, int64_t* n_total, int64_t* n_cells, int64_t* n_tiles);
 

@@ -105,6 +105,8 @@ SIGNATURES = {
     'rds_neighbor_counts': (C.c_int, [_VP, C.c_double, _VP]),
     'rds_time_kernels': (C.c_int, [_VP, C.c_int64, _VP, _VP]),
     'rds_debug_fill_lds': (C.c_int, [_VP, C.c_uint32]),
+    'rds_set_tiling': (C.c_int, [_VP, C.c_int32, C.c_int32]),
+    'rds_get_tiling': (C.c_int, [_VP, _VP, _VP, _VP]),
     'rds_sizes': (C.c_int, [_VP, _VP, _VP, _VP]),
     'rds_stream': (C.c_int, [_VP, C.POINTER(_VP)]),
 }

@@ -1,0 +1,74 @@
+"""Observables of the step loop's output: radial distribution function,
+activated fraction and cluster-size histogram.
+
+The reference computes none of these (it only writes trajectories); they are
+how the engine's precision modes are compared with the FP64 path:
+* RDF: pair histogram to r_max with bin dr, normalised by the ideal-gas
+  shell count N rho pi ((r+dr)^2 - r^2) / 2 per particle pair;
+* RDF error: || g - g_ref ||_1 / || g_ref ||_1 over the bins;
+* activated fraction: share of particles of a given type;
+* clusters: connected components of the given type with links shorter than
+  r_link (minimum image).
+"""
+
+import numpy as np
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import connected_components
+from scipy.spatial import cKDTree
+
+
+def _tree(x, L, periodic):
+    """KD-tree with periodic axes; a reflecting axis gets a box much larger
+    than the system, so that it never wraps."""
+    L = np.asarray(L, dtype=np.float64)
+    box = np.where(np.asarray(periodic) > 0, L, 4.0 * L + 100.0)
+    xx = np.asarray(x, dtype=np.float64).copy()
+    for k in range(2):
+        if periodic[k] > 0:
+            xx[:, k] = np.mod(xx[:, k], L[k])
+            xx[:, k][xx[:, k] >= L[k]] = 0.0
+    return cKDTree(xx, boxsize=box)
+
+
+def rdf(x, L, periodic, r_max=5.0, dr=0.02, select=None):
+    """g(r) at bin centres; `select` (bool mask) restricts to a species."""
+    x = np.asarray(x)
+    if select is not None:
+        x = x[select]
+    n = len(x)
+    edges = np.arange(0.0, r_max + dr / 2, dr)
+    if n < 2:
+        return 0.5 * (edges[1:] + edges[:-1]), np.zeros(len(edges) - 1)
+    tree = _tree(x, L, periodic)
+    pairs = tree.query_pairs(r_max, output_type='ndarray')
+    d = x[pairs[:, 1]] - x[pairs[:, 0]]
+    for k in range(2):
+        if periodic[k] > 0:
+            d[:, k] -= L[k] * np.round(d[:, k] / L[k])
+    r = np.sqrt((d ** 2).sum(axis=1))
+    counts, _ = np.histogram(r, bins=edges)
+    rho = n / (float(L[0]) * float(L[1]))
+    shell = np.pi * (edges[1:] ** 2 - edges[:-1] ** 2)
+    g = 2.0 * counts / (n * rho * shell)
+    return 0.5 * (edges[1:] + edges[:-1]), g
+
+
+def rdf_error(g, g_ref):
+    return float(np.abs(g - g_ref).sum() / np.abs(g_ref).sum())
+
+
+def activated_fraction(types, active=1):
+    return float(np.mean(np.asarray(types) == active))
+
+
+def cluster_sizes(x, types, L, periodic, active=1, r_link=1.5):
+    """Sizes of connected clusters of `active` particles (links < r_link)."""
+    sel = np.asarray(types) == active
+    xa = np.asarray(x)[sel]
+    n = len(xa)
+    if n == 0:
+        return np.zeros(0, dtype=int)
+    pairs = _tree(xa, L, periodic).query_pairs(r_link, output_type='ndarray')
+    m = coo_matrix((np.ones(len(pairs)), (pairs[:, 0], pairs[:, 1])), shape=(n, n))
+    _, lab = connected_components(m, directed=False)
+    return np.bincount(lab)

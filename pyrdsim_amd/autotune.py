@@ -1,0 +1,47 @@
+"""Autotuner for the force kernel's work decomposition.
+
+The tile shape -- cell rows per tile (one wavefront per row) and particles per
+row strip -- trades LDS staging overhead (bigger tiles stage fewer halo cells
+per particle) against parallelism, LDS occupancy and lane utilisation.  The
+best shape depends on the workload (density, replica sizes, GPU fill), so it is
+measured: each candidate runs a few real steps on the live state and the
+force kernel's HIP-event time decides.  Results are cached by a workload
+signature in a small JSON file.
+"""
+
+import json
+import os
+
+CANDIDATES = ((1, 40), (1, 56), (2, 40), (2, 56), (2, 80), (4, 40), (4, 56), (4, 96))
+
+
+def signature(engine):
+    """Workload key: precision, replica count, total particles, cell count."""
+    n, c, _ = engine.sizes()
+    return '%s/%d/%d/%d' % (engine.precision, len(engine.replicas), n, c)
+
+
+def autotune(engine, candidates=CANDIDATES, steps=4, cache=None):
+    """Pick (tile_rows, tile_particles) for `engine` by timing; leaves the
+    engine set to the best shape.  Returns (best, {shape: ms})."""
+    key = signature(engine)
+    table = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            table = json.load(f)
+        if key in table:
+            best = tuple(table[key]['best'])
+            engine.set_tiling(*best)
+            return best, {tuple(map(int, k.split('x'))): v for k, v in table[key]['ms'].items()}
+    times = {}
+    for rows, per in candidates:
+        engine.set_tiling(rows, per)
+        engine.time_kernels(1)                      # first launch of this variant
+        times[(rows, per)] = engine.time_kernels(steps)['force'][0]
+    best = min(times, key=times.get)
+    engine.set_tiling(*best)
+    if cache:
+        table[key] = dict(best=list(best), ms={'%dx%d' % k: v for k, v in times.items()})
+        with open(cache, 'w') as f:
+            json.dump(table, f, indent=1)
+    return best, times

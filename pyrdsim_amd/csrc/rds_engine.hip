@@ -1176,6 +1176,7 @@ struct rds_ctx {
   bool fast = false;    // all models: <= 2 types, one LJ term per pair, same even n
   int pw = 0;           // n / 2 of that LJ exponent
   int cap = 384;        // staged particles per LDS round
+  int target = 56;      // particles per tile row strip
   bool lists = false;   // two-pass force evaluation (partner lists in LDS)
   size_t smem_bytes = 0;
 };
@@ -1445,11 +1446,79 @@ int fill_model(const rds_model& m, DModel& d, double* rcut) {
   return RDS_OK;
 }
 
+// Tiles of `rows` cell rows x W cells with W chosen so that a row strip holds
+// about `target` particles at the replica's mean occupancy, and the LDS round
+// capacity: 1.5x the mean staged count of the densest tile shape, at least
+// 256, at most what 64 KB of dynamic LDS holds (denser tiles stage in rounds).
+bool build_tiles(rds_ctx* c, int rows, int target) {
+  if (rows != 1 && rows != 2 && rows != 4) {
+    fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4");
+    return false;
+  }
+  if (target <= 0) target = 56;
+  c->rows = rows;
+  c->h_tiles.clear();
+  double staged_worst = 0.0;
+  for (int r = 0; r < c->n_rep; ++r) {
+    const DRep& d = c->h_rep[r];
+    double occ = (double)d.n / ((double)d.lx * d.ly);
+    int W = (int)std::floor((double)target / std::max(occ, 1e-9));
+    W = std::max(1, std::min(W, std::min(MAXW, d.lx)));
+    int nstrip = (d.lx + W - 1) / W;
+    int Wq = d.lx / nstrip, Wr = d.lx % nstrip;
+    staged_worst = std::max(staged_worst, occ * (rows + 2) * (Wq + (Wr ? 1 : 0) + 2));
+    for (int y = 0; y < d.ly; y += rows) {
+      int nr = std::min(rows, d.ly - y);
+      int x0 = 0;
+      for (int k = 0; k < nstrip; ++k) {
+        int wk = Wq + (k < Wr ? 1 : 0);
+        c->h_tiles.push_back(make_int4(r | ((nr - 1) << 24), y, x0, wk));
+        x0 += wk;
+      }
+    }
+  }
+  c->n_tiles = (int64_t)c->h_tiles.size();
+  const int per = 16 + 4 + (c->has_mol ? 4 : 0);
+  int cap = (int)std::ceil(1.5 * staged_worst / 64.0) * 64;
+  cap = std::max(cap, 256);
+  cap = std::min(cap, (65536 - 256) / per / 64 * 64);
+  c->cap = cap;
+  c->smem_bytes = (size_t)cap * per;
+  return true;
+}
+
+int upload_tiles(rds_ctx* c) {
+  if (c->d_tiles) hipFree(c->d_tiles);
+  c->d_tiles = nullptr;
+  RDS_TRY(dalloc(&c->d_tiles, (size_t)c->n_tiles));
+  HIP_TRY(hipMemcpy(c->d_tiles, c->h_tiles.data(), sizeof(int4) * c->n_tiles,
+                    hipMemcpyHostToDevice));
+  return RDS_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 int rds_abi_version(void) { return RDS_ABI_VERSION; }
+
+int rds_set_tiling(rds_ctx* c, int32_t tile_rows, int32_t tile_particles) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (!build_tiles(c, tile_rows, tile_particles)) return RDS_E_ARG;
+  RDS_TRY(upload_tiles(c));
+  c->target = tile_particles > 0 ? tile_particles : 56;
+  if (c->loaded) RDS_TRY(build_graph(c));
+  return RDS_OK;
+}
+
+int rds_get_tiling(rds_ctx* c, int32_t* tile_rows, int32_t* tile_particles, int32_t* lds_cap) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  if (tile_rows) *tile_rows = c->rows;
+  if (tile_particles) *tile_particles = c->target;
+  if (lds_cap) *lds_cap = c->cap;
+  return RDS_OK;
+}
 
 const char* rds_last_error(void) { return g_err.c_str(); }
 
@@ -1498,6 +1567,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   c->lists = (cfg->cells_per_cutoff < 0);
   if (c->rows != 1 && c->rows != 2 && c->rows != 4)
     return bail(fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4"));
+  c->target = cfg->tile_particles > 0 ? cfg->tile_particles : 56;
   if (cfg->n_replicas >= (1 << 24)) return bail(fail(RDS_E_ARG, "too many replicas"));
   const int spc = cfg->cells_per_cutoff != 0 ? std::abs(cfg->cells_per_cutoff) : 1;
   if (spc != 1) return bail(fail(RDS_E_ARG, "only cells_per_cutoff = 1 (3x3 stencil) is built"));
@@ -1529,7 +1599,6 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(RDS_E_HIP, "no HIP device"));
   c->h_rep.resize(c->n_rep);
   int64_t pbase = 0, cbase = 0;
-  double staged_worst = 0.0;
   for (int r = 0; r < c->n_rep; ++r) {
     const rds_replica& s = cfg->replicas[r];
     DRep& d = c->h_rep[r];
@@ -1576,22 +1645,6 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     }
     for (int q = 0; q < MAXRX; ++q) d.rate[q] = s.rate[q];
     d.dt = s.dt;
-    // tiles: strips of W cells per row
-    double occ = (double)s.n_particles / ((double)d.lx * d.ly);
-    int W = (int)std::floor(target / std::max(occ, 1e-9));
-    W = std::max(1, std::min(W, std::min(MAXW, d.lx)));
-    int nstrip = (d.lx + W - 1) / W;
-    int Wq = d.lx / nstrip, Wr = d.lx % nstrip;
-    staged_worst = std::max(staged_worst, occ * (c->rows + 2) * (Wq + (Wr ? 1 : 0) + 2));
-    for (int y = 0; y < d.ly; y += c->rows) {
-      int nr = std::min(c->rows, d.ly - y);
-      int x0 = 0;
-      for (int k = 0; k < nstrip; ++k) {
-        int wk = Wq + (k < Wr ? 1 : 0);
-        c->h_tiles.push_back(make_int4(r | ((nr - 1) << 24), y, x0, wk));
-        x0 += wk;
-      }
-    }
     pbase += s.n_particles;
     cbase += (int64_t)d.lx * d.ly;
     if (pbase > 0x7fffffff || cbase > 0x7ffffff0)
@@ -1599,18 +1652,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   }
   c->n_total = pbase;
   c->n_cells = cbase;
-  c->n_tiles = (int64_t)c->h_tiles.size();
-  // LDS round capacity: 1.5x the mean staged count of the densest tile shape,
-  // at least 256, at most what 64 KB of dynamic LDS holds; denser tiles stage
-  // in several rounds
-  {
-    const int per = 16 + 4 + (c->has_mol ? 4 : 0);
-    int cap = (int)std::ceil(1.5 * staged_worst / 64.0) * 64;
-    cap = std::max(cap, 256);
-    cap = std::min(cap, (65536 - 256) / per / 64 * 64);
-    c->cap = cap;
-    c->smem_bytes = (size_t)cap * per;
-  }
+  if (!build_tiles(c, c->rows, (int)target)) return bail(RDS_E_ARG);
 
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t N = (size_t)c->n_total, C = (size_t)c->n_cells;
@@ -1618,7 +1660,6 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   if ((rc = dalloc(&(p), (n))) != RDS_OK) return bail(rc)
   A_(c->d_rep, c->n_rep);
   A_(c->d_models, c->n_models);
-  A_(c->d_tiles, (size_t)c->n_tiles);
   A_(c->d_rep_of, N);
   for (int k = 0; k < 2; ++k) {
     A_(c->buf[k].pos, N);
@@ -1638,7 +1679,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
 
   HIP_TRY(hipMemcpy(c->d_rep, c->h_rep.data(), sizeof(DRep) * c->n_rep, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_models, c->h_models.data(), sizeof(DModel) * c->n_models, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(c->d_tiles, c->h_tiles.data(), sizeof(int4) * c->n_tiles, hipMemcpyHostToDevice));
+  if ((rc = upload_tiles(c)) != RDS_OK) return bail(rc);
   {
     std::vector<int> rep_of(N);
     std::vector<int4> sb;

@@ -291,6 +291,14 @@ class Engine:
         self.step += int(n_steps)
         return {k: (float(avg[i]), int(cnt[i])) for i, k in enumerate(L.KERNELS)}
 
+    def set_tiling(self, tile_rows, tile_particles):
+        L.check(self.lib.rds_set_tiling(self.ctx, int(tile_rows), int(tile_particles)))
+
+    def tiling(self):
+        r, t, c = C.c_int32(), C.c_int32(), C.c_int32()
+        L.check(self.lib.rds_get_tiling(self.ctx, C.byref(r), C.byref(t), C.byref(c)))
+        return r.value, t.value, c.value
+
     def debug_fill_lds(self, pattern=0x7fc00001):
         """Test hook: fill LDS with a (NaN) pattern before the next launch."""
         L.check(self.lib.rds_debug_fill_lds(self.ctx, int(pattern) & 0xffffffff))

@@ -1,0 +1,60 @@
+"""Precision modes and tuning on the GPU.
+
+* FP32 pair forces with FP64 positions reproduce the FP64 path's radial
+  distribution function within 1% (L1, relative) and its activated fraction,
+  averaged over the second half of a run, with the same Philox streams;
+* re-tiling a live context (what the autotuner does) changes nothing but the
+  work decomposition.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import engine_for, small
+from pyrdsim_amd import analysis
+from pyrdsim_amd.autotune import autotune
+
+pytestmark = pytest.mark.gpu
+
+
+def _averaged(eng, s, steps, every):
+    gs, acts = [], []
+    for k in range(steps // every):
+        eng.run(every)
+        if k >= (steps // every) // 2:
+            x, t = eng.state()
+            _, g = analysis.rdf(x, s['L'], s['periodic'])
+            gs.append(g)
+            acts.append(analysis.activated_fraction(t))
+    return np.mean(gs, axis=0), float(np.mean(acts))
+
+
+@pytest.mark.parametrize('name', ['kalj', 'wall'])
+def test_mixed_precision_rdf_within_one_percent(gpu, name):
+    s = small(name, n=6000, seed=21)
+    steps, every = 4000, 200
+    res = {}
+    for prec in ('fp64', 'mixed'):
+        eng = engine_for(s, precision=prec, seed=77, graph_steps=20)
+        res[prec] = _averaged(eng, s, steps, every)
+        eng.close()
+    err = analysis.rdf_error(res['mixed'][0], res['fp64'][0])
+    assert err <= 0.01, err
+    assert abs(res['mixed'][1] - res['fp64'][1]) <= 0.01
+
+
+def test_retiling_keeps_state_and_forces(gpu):
+    s = small('wall', n=5000, seed=3)
+    s['types'][::4] = 1
+    eng = engine_for(s, precision='fp64', seed=4)
+    eng.run(10)
+    x0, t0 = eng.state()
+    f0 = eng.forces()
+    for rows, per in ((2, 40), (4, 96), (1, 8)):
+        eng.set_tiling(rows, per)
+        assert eng.tiling()[:2] == (rows, per)
+        x1, t1 = eng.state()
+        assert np.array_equal(x0, x1) and np.array_equal(t0, t1)
+        assert np.abs(eng.forces() - f0).max() <= 1e-12 * (np.abs(f0).max() + 1)
+    best, times = autotune(eng, steps=2)
+    assert best in times and eng.tiling()[:2] == best
