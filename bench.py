@@ -184,7 +184,6 @@ def main():
     ap.add_argument('--graph-steps', type=int, default=50)
     ap.add_argument('--tile-particles', type=int, default=0)
     ap.add_argument('--tile-rows', type=int, default=1)
-    ap.add_argument('--lists', action='store_true')
     ap.add_argument('--ic-cache', default=None, help='.npz file of initial states (read or written)')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-autotune', dest='autotune', action='store_false')
@@ -220,8 +219,7 @@ def main():
     t0s = np.concatenate([s[1] for s in states])
 
     eng = Engine(models, reps, precision='mixed', graph_steps=args.graph_steps,
-                 tile_particles=args.tile_particles, tile_rows=args.tile_rows,
-                 lists=args.lists)
+                 tile_particles=args.tile_particles, tile_rows=args.tile_rows)
     eng.set_state(x0, t0s)
     tuned = None
     if args.autotune:

@@ -46,7 +46,6 @@
 #define MAXWALLS RDS_MAX_WALLS
 #define MAXRX RDS_MAX_REACTIONS
 #define MAXW 60        // cells per tile strip
-#define LMAX 40        // per-lane partner list of the force kernel
 
 namespace {
 
@@ -526,7 +525,7 @@ __device__ __forceinline__ Real lj_fast(Real r2, Real pa, Real pb, int pw) {
   return pa * inv * s * (s - Real(1));
 }
 
-template <typename Real, bool MOL, int KM, int R, bool LISTS>
+template <typename Real, bool MOL, int KM, int R>
 __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, int pw) {
   constexpr bool FAST = KM != 0;
   extern __shared__ __align__(16) char smem[];
@@ -538,7 +537,6 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
   __shared__ TileSh<R> sh;
   __shared__ int s_pfirst[FAST ? 1 : MAXT * MAXT], s_pcount[FAST ? 1 : MAXT * MAXT];
   __shared__ DTerm s_term[FAST ? 1 : MAXTERMS];
-  __shared__ unsigned short s_list[LISTS ? R * 64 * LMAX : 1];  // per-lane partner lists
 
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
@@ -621,54 +619,25 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
         }
       }
     };
-    const double rc2max = M.rc2max;
-    unsigned short* my_list = s_list + wv * 64 * LMAX + lane;  // entry k at my_list[64 k]
     for (int p0 = 0; p0 < total; p0 += CAPT) {
       if (!resident) {
         __syncthreads();
         tile_stage<R, MOL>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
         __syncthreads();
       }
-      if (!LISTS) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const int lo = max(h.lo[d], p0) - p0;
-          const int hi = min(h.hi[d], p0 + CAPT) - p0;
-          // two candidates per trip; past the window end the second one
-          // repeats j with its contribution masked (it must read a staged
-          // element: 0 * dx of an unstaged one could be 0 * NaN)
-          for (int j = lo; j < hi; j += 2) {
-            pair(j, p0, true);
-            const bool two = j + 1 < hi;
-            pair(two ? j + 1 : j, p0, two);
-          }
-        }
-        continue;
-      }
-      // pass A: FP64 distance test over the 3x3 stencil; partners within the
-      // model's largest cutoff go to this lane's list (in window order)
-      int cnt = 0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const int lo = max(h.lo[d], p0) - p0;
         const int hi = min(h.hi[d], p0 + CAPT) - p0;
-        for (int j = lo; j < hi; ++j) {
-          const double2 pj = s_pos[j];
-          const double ddx = pj.x - xi.x, ddy = pj.y - xi.y;
-          const double r2 = ddx * ddx + ddy * ddy;
-          const bool other = MOL ? (s_mol[j] != mi) : (r2 > 0.0);
-          if (other && r2 <= rc2max) {
-            if (cnt < LMAX) {
-              my_list[64 * cnt] = (unsigned short)j;
-              ++cnt;
-            } else {
-              pair(j, p0, true);  // list full: evaluate in place
-            }
-          }
+        // two candidates per trip; past the window end the second one
+        // repeats j with its contribution masked (it must read a staged
+        // element: 0 * dx of an unstaged one could be 0 * NaN)
+        for (int j = lo; j < hi; j += 2) {
+          pair(j, p0, true);
+          const bool two = j + 1 < hi;
+          pair(two ? j + 1 : j, p0, two);
         }
       }
-      // pass B: the pair laws over the list only
-      for (int k = 0; k < cnt; ++k) pair(my_list[64 * k], p0, true);
     }
     if (!h.valid) continue;
     const int s = h.s;
@@ -1177,7 +1146,6 @@ struct rds_ctx {
   int pw = 0;           // n / 2 of that LJ exponent
   int cap = 384;        // staged particles per LDS round
   int target = 56;      // particles per tile row strip
-  bool lists = false;   // two-pass force evaluation (partner lists in LDS)
   size_t smem_bytes = 0;
 };
 
@@ -1217,24 +1185,22 @@ StepArgs make_args(rds_ctx* c, const Buf& cur) {
   return a;
 }
 
-template <typename Real, bool MOL, int KM, bool LISTS>
+template <typename Real, bool MOL, int KM>
 void launch_force_r(rds_ctx* c, const StepArgs& a, int mode) {
   dim3 g((unsigned)c->n_tiles);
   const size_t sm = c->smem_bytes;
   switch (c->rows) {
-    case 4: hipLaunchKernelGGL((k_force<Real, MOL, KM, 4, LISTS>), g, dim3(256), sm, c->stream, a, mode, c->pw); break;
-    case 2: hipLaunchKernelGGL((k_force<Real, MOL, KM, 2, LISTS>), g, dim3(128), sm, c->stream, a, mode, c->pw); break;
-    default: hipLaunchKernelGGL((k_force<Real, MOL, KM, 1, LISTS>), g, dim3(64), sm, c->stream, a, mode, c->pw); break;
+    case 4: hipLaunchKernelGGL((k_force<Real, MOL, KM, 4>), g, dim3(256), sm, c->stream, a, mode, c->pw); break;
+    case 2: hipLaunchKernelGGL((k_force<Real, MOL, KM, 2>), g, dim3(128), sm, c->stream, a, mode, c->pw); break;
+    default: hipLaunchKernelGGL((k_force<Real, MOL, KM, 1>), g, dim3(64), sm, c->stream, a, mode, c->pw); break;
   }
 }
 
 template <typename Real, bool MOL>
 void launch_force_m(rds_ctx* c, const StepArgs& a, int mode) {
-  if (!c->fast) launch_force_r<Real, MOL, 0, false>(c, a, mode);
-  else if (c->pw == 3) {
-    if (c->lists) launch_force_r<Real, MOL, 2, true>(c, a, mode);
-    else launch_force_r<Real, MOL, 2, false>(c, a, mode);
-  } else launch_force_r<Real, MOL, 1, false>(c, a, mode);
+  if (!c->fast) launch_force_r<Real, MOL, 0>(c, a, mode);
+  else if (c->pw == 3) launch_force_r<Real, MOL, 2>(c, a, mode);
+  else launch_force_r<Real, MOL, 1>(c, a, mode);
 }
 
 template <typename Real>
@@ -1564,7 +1530,6 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   c->n_rep = cfg->n_replicas;
   c->n_models = cfg->n_models;
   c->rows = cfg->tile_rows > 0 ? cfg->tile_rows : 1;
-  c->lists = (cfg->cells_per_cutoff < 0);
   if (c->rows != 1 && c->rows != 2 && c->rows != 4)
     return bail(fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4"));
   c->target = cfg->tile_particles > 0 ? cfg->tile_particles : 56;

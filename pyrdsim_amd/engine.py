@@ -129,7 +129,7 @@ class Engine:
 
     def __init__(self, models, replicas, precision='mixed', rng='philox',
                  reaction_prob='linear', bonds=None, has_molecules=False,
-                 tile_particles=0, graph_steps=0, tile_rows=1, lists=False, device='cuda'):
+                 tile_particles=0, graph_steps=0, tile_rows=1, device='cuda'):
         self.lib = L.load()
         if not torch.cuda.is_available():
             raise RuntimeError('the engine needs a ROCm GPU; none is visible')
@@ -151,7 +151,7 @@ class Engine:
         cfg.reaction_prob = PROBS[reaction_prob]
         cfg.has_molecules = 1 if has_molecules else 0
         cfg.tile_particles = int(tile_particles)
-        cfg.cells_per_cutoff = -1 if lists else 1
+        cfg.cells_per_cutoff = 1
         cfg.graph_steps = int(graph_steps)
         cfg.tile_rows = int(tile_rows)
         self._bond_arrays = None
