@@ -35,6 +35,10 @@ from pyrdsim_amd.autotune import autotune  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FORCE_BYTES_PER_PARTICLE = 48   # read pos 16 + type 4 + pid 4; write pos 16 + cell 4 + rank 4
 FORCE_BYTES_PER_CELL = 4        # cell starts
+# list mode: read pos 16 + type 4 + pid 4 + reference pos 16 + sorted cell 4 + list length 2,
+# write pos 16; plus 2 bytes per list entry
+LIST_BYTES_PER_PARTICLE = 62
+LIST_BYTES_PER_ENTRY = 2
 
 
 def _make_replica_state(args):
@@ -184,6 +188,8 @@ def main():
     ap.add_argument('--graph-steps', type=int, default=50)
     ap.add_argument('--tile-particles', type=int, default=0)
     ap.add_argument('--tile-rows', type=int, default=1)
+    ap.add_argument('--skin', type=float, default=0.0,
+                    help='Verlet skin of the batched engine (0: cell search every step)')
     ap.add_argument('--ic-cache', default=None, help='.npz file of initial states (read or written)')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-autotune', dest='autotune', action='store_false')
@@ -219,23 +225,36 @@ def main():
     t0s = np.concatenate([s[1] for s in states])
 
     eng = Engine(models, reps, precision='mixed', graph_steps=args.graph_steps,
-                 tile_particles=args.tile_particles, tile_rows=args.tile_rows)
+                 tile_particles=args.tile_particles, tile_rows=args.tile_rows, skin=args.skin)
     eng.set_state(x0, t0s)
     tuned = None
     if args.autotune:
         best, times = autotune(eng)
         tuned = dict(best=list(best), force_ms={'%dx%d' % k: round(v, 5) for k, v in times.items()})
         print('# autotune: rows x particles = %s' % (best,), file=sys.stderr, flush=True)
+    st0 = eng.sync()
     secs = time_engine(eng, args.steps, args.warmup)
+    st1 = eng.sync()
     value = n_tot * args.steps / secs
     print('# batched: %.3e particle-steps/s (%.3f ms/step)' % (value, 1e3 * secs / args.steps),
           file=sys.stderr, flush=True)
 
     kt = eng.time_kernels(20)
     eng.sync()
+    st2 = eng.sync()
     _, n_cells, n_tiles = eng.sizes()
     force_ms = kt['force'][0]
-    force_bytes = FORCE_BYTES_PER_PARTICLE * n_tot + FORCE_BYTES_PER_CELL * n_cells
+    lists = None
+    if args.skin > 0:
+        mean_len = st2.list_entries / max(1, st2.rebuilds) / n_tot
+        force_bytes = (LIST_BYTES_PER_PARTICLE + LIST_BYTES_PER_ENTRY * mean_len) * n_tot + \
+            FORCE_BYTES_PER_CELL * n_cells
+        lists = dict(skin=args.skin, mean_list_length=round(mean_len, 2),
+                     steps_per_rebuild=round((args.steps + args.warmup) /
+                                             max(1, st1.rebuilds - st0.rebuilds), 2),
+                     overflow=int(st2.list_overflow))
+    else:
+        force_bytes = FORCE_BYTES_PER_PARTICLE * n_tot + FORCE_BYTES_PER_CELL * n_cells
     achieved = force_bytes / (force_ms * 1e-3) / 1e9
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
@@ -309,7 +328,7 @@ def main():
                         'sum N=%d' % (len(plan), n_tot),
             'precision': 'mixed', 'graph_steps': args.graph_steps,
             'tiling': list(tiling[:2]), 'lds_cap': tiling[2], 'autotune': tuned,
-            'cells': int(n_cells), 'tiles': int(n_tiles)},
+            'cells': int(n_cells), 'tiles': int(n_tiles), 'lists': lists},
         'roofline': roofline,
         'plain': plain,
         'batched_vs_plain': (value / plain['value']) if plain else None,

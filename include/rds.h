@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define RDS_ABI_VERSION 1
+#define RDS_ABI_VERSION 2
 
 #define RDS_MAX_TYPES 8
 #define RDS_MAX_TERMS 32
@@ -127,6 +127,19 @@ typedef struct {
   const int32_t* bond_j;
   const double* bond_r0;
   const double* bond_u0;
+  /* Neighbour search.  skin = 0: every step re-sorts the particles by cell and
+   * the force kernel scans the 3x3 cell stencil.  skin > 0 (Verlet lists, as
+   * the reference's create_neighbor_list with rs = range + skin,
+   * pyrdsim.pyx:1448-1587): cells are at least (range + skin) wide, and each
+   * particle keeps a list of the partners within range + skin.  The sort and
+   * the lists are rebuilt in the step after some particle of the context has
+   * moved more than skin/2 from where it was at the last rebuild, in place of
+   * the reference's fixed neighbor_list_update_frequency; every pair within
+   * range is therefore always listed.  list_capacity: entries per particle
+   * (0 = 64); a particle with more partners falls back to the stencil scan. */
+  double skin;
+  int32_t list_capacity;
+  int32_t reserved0;
 } rds_config;
 
 /* Result of the last rds_sync. */
@@ -141,6 +154,9 @@ typedef struct {
   int64_t reaction_events;  /* type changes since set_state */
   int32_t integrity;        /* nonzero: an internal cell-list invariant failed */
   int32_t reserved;
+  int64_t rebuilds;         /* list mode: sorts + list builds since set_state */
+  int64_t list_overflow;    /* list mode: particles that exceeded list_capacity */
+  int64_t list_entries;     /* list mode: sum of list lengths over all builds */
 } rds_status;
 
 typedef struct rds_ctx rds_ctx;
@@ -198,7 +214,9 @@ int rds_neighbor_counts(rds_ctx* ctx, double r, int32_t* counts);
 
 /* Times n_steps eager steps with HIP events around every launch on the
  * context's stream; avg_ms[k] for k = 0 force+integrate, 1 scan, 2 scatter,
- * 3 reaction, 4 bonds; counts[k] = launches timed. */
+ * 3 reaction, 4 bonds, 5 list mode's binning + list build (the conditional
+ * kernels are timed on every step, also when they have nothing to do);
+ * counts[k] = launches timed. */
 int rds_time_kernels(rds_ctx* ctx, int64_t n_steps, double* avg_ms, int64_t* counts);
 
 /* Test hook with no counterpart in the reference: fills the LDS of many

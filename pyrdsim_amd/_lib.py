@@ -14,7 +14,7 @@ MAX_TYPES = 8
 MAX_TERMS = 32
 MAX_WALLS = 8
 MAX_REACTIONS = 8
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK, E_ARG, E_HIP, E_BOUNDS, E_STATE, E_STREAM = 0, -1, -2, -3, -4, -5
 FP64, MIXED = 0, 1
@@ -29,7 +29,7 @@ REACTION_KINDS = {'linear': RX_LINEAR, 'locally_catalyzed': RX_CATALYZED,
                   'locally_inhibited': RX_INHIBITED}
 
 # names of the timed kernels of rds_time_kernels
-KERNELS = ('force', 'scan', 'scatter', 'react', 'bonds')
+KERNELS = ('force', 'scan', 'scatter', 'react', 'bonds', 'list')
 
 
 class PairTerm(C.Structure):
@@ -75,7 +75,8 @@ class Config(C.Structure):
                 ('graph_steps', C.c_int32), ('tile_rows', C.c_int32),
                 ('n_bonds', C.c_int64), ('bond_i', C.POINTER(C.c_int32)),
                 ('bond_j', C.POINTER(C.c_int32)), ('bond_r0', C.POINTER(C.c_double)),
-                ('bond_u0', C.POINTER(C.c_double))]
+                ('bond_u0', C.POINTER(C.c_double)), ('skin', C.c_double),
+                ('list_capacity', C.c_int32), ('reserved0', C.c_int32)]
 
 
 class Status(C.Structure):
@@ -83,7 +84,9 @@ class Status(C.Structure):
                 ('stream_error', C.c_int32), ('error_step', C.c_int64),
                 ('error_particle', C.c_int32), ('error_axis', C.c_int32),
                 ('error_value', C.c_double), ('reaction_events', C.c_int64),
-                ('integrity', C.c_int32), ('reserved', C.c_int32)]
+                ('integrity', C.c_int32), ('reserved', C.c_int32),
+                ('rebuilds', C.c_int64), ('list_overflow', C.c_int64),
+                ('list_entries', C.c_int64)]
 
 
 # every entry point of include/rds.h: name -> (restype, argtypes)

@@ -12,7 +12,8 @@ signature in a small JSON file.
 import json
 import os
 
-CANDIDATES = ((1, 40), (1, 56), (2, 40), (2, 56), (2, 80), (4, 40), (4, 56), (4, 96))
+CANDIDATES = ((1, 40), (1, 56), (2, 40), (2, 56), (2, 80), (2, 112), (4, 40), (4, 56), (4, 96),
+              (4, 144))
 
 
 def signature(engine):

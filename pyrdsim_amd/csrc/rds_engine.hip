@@ -109,6 +109,7 @@ struct DRep {
   int sx, sy;             // periodic images in the cell stencil
   double fpre[MAXT], npre[MAXT], rate[MAXRX];
   double dt;
+  double rl2;             // list mode: (largest pair cutoff + skin)^2
 };
 
 struct DStatus {
@@ -117,8 +118,13 @@ struct DStatus {
   int bounds, stream;
   unsigned long long events;
   int integrity;               // bit 0: scatter index out of range, bit 1: histogram total
+  int rebuild;                 // list mode: set when a particle moved more than skin/2
+  unsigned long long rebuilds, overflow, entries;
+  unsigned done;               // blocks of the list build that finished (reset by the last)
   int pad;
 };
+
+#define LIST_OVER 0xFFFF  // list length marking a particle that uses the stencil scan
 
 // Everything a kernel of one step needs.  Passed by value.
 struct StepArgs {
@@ -159,6 +165,15 @@ struct StepArgs {
   int cap;
   const int* rep_of;  // replica of each pid
   const int4* sblocks;  // (replica, first slot, end slot) blocks of <= 256 slots
+  // list mode
+  uint2* nl4;           // [K/4][n_total] staged indices of each slot's partners,
+                        // four 16-bit entries per chunk, the last chunk padded
+                        // with its first entry
+  unsigned short* ncnt; // [n_total] list length, or LIST_OVER
+  int K;                // multiple of 4
+  double2* xref;        // position of each slot at the last rebuild
+  const int* scell;     // cell of each slot at the last rebuild (null: cell mode)
+  double hskin2;        // (skin / 2)^2
 };
 
 // ---------------------------------------------------------------------------
@@ -339,11 +354,33 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
   __syncthreads();
 }
 
-// Stages staged indices [p0, p0 + cap) into LDS.
+// List mode: a periodic coordinate may have wrapped since the slot was sorted
+// into `cell`; the image meant by the stencil is the one nearest that cell's
+// centre (unambiguous: cells are >= range + skin wide and the particle has
+// moved less than skin/2).
+__device__ __forceinline__ double2 near_cell(double2 q, const DRep& Rp, int cell) {
+  const int c = cell - Rp.cbase;
+  const int cy = c / Rp.lx, cx = c - cy * Rp.lx;
+  if (Rp.px) {
+    const double d = q.x - ((double)cx + 0.5) * Rp.cl[0];
+    if (d > 0.5 * Rp.L[0]) q.x -= Rp.L[0];
+    else if (d < -0.5 * Rp.L[0]) q.x += Rp.L[0];
+  }
+  if (Rp.py) {
+    const double d = q.y - ((double)cy + 0.5) * Rp.cl[1];
+    if (d > 0.5 * Rp.L[1]) q.y -= Rp.L[1];
+    else if (d < -0.5 * Rp.L[1]) q.y += Rp.L[1];
+  }
+  return q;
+}
+
+// Stages staged indices [p0, p0 + cap) into LDS.  scell (list mode): cells of
+// the last rebuild, whose nearest images are staged.
 template <int R, bool MOL>
 __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
                            const double2* __restrict__ pos, const int* __restrict__ type,
-                           const int* __restrict__ mol, double2* s_pos, int* s_ty, int* s_mol) {
+                           const int* __restrict__ mol, double2* s_pos, int* s_ty, int* s_mol,
+                           const DRep& Rp, const int* __restrict__ scell) {
   const int t = threadIdx.x, nthr = blockDim.x;
   const int p1 = p0 + cap;
   for (int r = 0; r < NR; ++r) {
@@ -357,6 +394,7 @@ __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
       const int sl = sh.seg_slot[r][g] - so;
       for (int p = a + t; p < b; p += nthr) {
         double2 q = pos[sl + p];
+        if (scell) q = near_cell(q, Rp, scell[sl + p]);
         q.x = q.x + sx;
         q.y = q.y + sy;
         s_pos[p - p0] = q;
@@ -368,6 +406,8 @@ __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
 }
 
 // Home particle of lane `lane` of wave q in batch b, and its three windows.
+// The home cell is the one the slot was sorted into: scell (list mode, cells of
+// the last rebuild) or, with scell null, the cell of the current position.
 template <int R>
 struct Home {
   int s;
@@ -379,7 +419,8 @@ struct Home {
 template <int R>
 __device__ __forceinline__ void home_windows(const TileSh<R>& sh, int q, int b, int lane,
                                              int nrows, const DRep& Rp, int cx0, int W,
-                                             const double2* __restrict__ pos, Home<R>& h,
+                                             const double2* __restrict__ pos,
+                                             const int* __restrict__ scell, Home<R>& h,
                                              double2& xi) {
   const int hr = q + 1;  // staged row of the home row
   const bool rowok = q < nrows;
@@ -392,7 +433,13 @@ __device__ __forceinline__ void home_windows(const TileSh<R>& sh, int q, int b, 
   h.own = -1;
   if (h.valid) {
     xi = pos[h.s];
-    int cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
+    int cx;
+    if (scell) {
+      const int c = scell[h.s] - Rp.cbase;
+      cx = c - (c / Rp.lx) * Rp.lx;
+    } else {
+      cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
+    }
     h.w = min(max(cx - cx0, 0), W - 1);
     h.own = sh.row_base[hr] + sh.off[hr][h.w + 1] + (h.s - sh.cs[hr][h.w + 1]);
   }
@@ -525,7 +572,7 @@ __device__ __forceinline__ Real lj_fast(Real r2, Real pa, Real pb, int pw) {
   return pa * inv * s * (s - Real(1));
 }
 
-template <typename Real, bool MOL, int KM, int R>
+template <typename Real, bool MOL, int KM, int R, bool LIST>
 __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, int pw) {
   constexpr bool FAST = KM != 0;
   extern __shared__ __align__(16) char smem[];
@@ -548,6 +595,7 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
   const int cy0 = T.y, cx0 = T.z, W = T.w;
   const int nt = M.nt;
   const int NR = nrows + 2;
+  const size_t NS = (size_t)A.n_total;
 
   if (!FAST) {
     for (int q = t; q < nt * nt; q += blockDim.x) {
@@ -559,8 +607,9 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
   tile_setup<R>(Rp, A.cstart, cy0, cx0, W, nrows, sh);
   const int total = sh.total;
   const bool resident = total <= CAPT;
+  const int* scell = LIST ? A.scell : nullptr;
   if (resident) {
-    tile_stage<R, MOL>(sh, NR, 0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+    tile_stage<R, MOL>(sh, NR, 0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, scell);
     __syncthreads();
   }
   long long step_next = 0;
@@ -569,12 +618,28 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
   for (int b = 0; b < sh.nbatch; ++b) {
     Home<R> h;
     double2 xi;
-    home_windows<R>(sh, wv, b, lane, nrows, Rp, cx0, W, A.pos, h, xi);
+    home_windows<R>(sh, wv, b, lane, nrows, Rp, cx0, W, A.pos, scell, h, xi);
     int ti = 0, mi = 0, p = 0;
+    // position in the frame of the staged tile (list mode: nearest image to
+    // the sorted cell); xi itself stays the integrator's coordinate
+    double2 xs = xi;
     if (h.valid) {
       ti = A.type[h.s];
       p = A.pid[h.s];
       if (MOL) mi = A.mol[h.s];
+      if (LIST) xs = near_cell(xi, Rp, scell[h.s]);
+    }
+    // list mode: the lane walks its partner list; a lane whose list overflowed
+    // scans the stencil windows instead (the other lanes' windows are empty)
+    int nlist = 0;
+    if (LIST && h.valid) {
+      nlist = A.ncnt[h.s];
+      if (nlist != LIST_OVER) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) h.lo[d] = h.hi[d] = 0;
+      } else {
+        nlist = 0;
+      }
     }
     // FAST: the (ti, tj) constants for tj = 0, 1 in registers
     Real pa0 = 0, pb0 = 0, pc0 = -1, pa1 = 0, pb1 = 0, pc1 = -1;
@@ -591,15 +656,14 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
     }
     Real fx = 0, fy = 0;
     const int tbase = ti * nt;
-    // one candidate: staged index j, counted when `live`
-    auto pair = [&](int j, int p0, bool live) {
-      const double2 pj = s_pos[j];
-      const int tj = s_ty[j];
-      const Real dx = (Real)(pj.x - xi.x), dy = (Real)(pj.y - xi.y);
+    // one candidate: staged index j, counted when `live`.  Listed partners are
+    // never the particle itself nor of its molecule; stencil candidates are
+    // filtered here: self / same molecule never interact, and a particle's only
+    // zero-distance partner in an atomic model is itself.
+    auto pairv = [&](const double2 pj, int tj, int mj, bool live, bool listed) {
+      const Real dx = (Real)(pj.x - xs.x), dy = (Real)(pj.y - xs.y);
       const Real r2 = dx * dx + dy * dy;
-      // self / same molecule: pairs of one molecule never interact; a
-      // particle's only zero-distance partner in an atomic model is itself
-      const bool other = MOL ? (s_mol[j] != mi) : (r2 > Real(0));
+      const bool other = (MOL && listed) ? true : (MOL ? (mj != mi) : (r2 > Real(0)));
       if (FAST) {
         const bool m1 = tj == 1;
         const Real pa = m1 ? pa1 : pa0, pb = m1 ? pb1 : pb0, pc = m1 ? pc1 : pc0;
@@ -619,11 +683,44 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
         }
       }
     };
+    auto pair = [&](int j, bool live, bool listed) {
+      pairv(s_pos[j], s_ty[j], MOL ? s_mol[j] : 0, live, listed);
+    };
+    const uint2* lp = A.nl4 + (LIST && h.valid ? h.s : 0);
+    int kcur = 0;  // list cursor across LDS rounds
     for (int p0 = 0; p0 < total; p0 += CAPT) {
       if (!resident) {
         __syncthreads();
-        tile_stage<R, MOL>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+        tile_stage<R, MOL>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, scell);
         __syncthreads();
+      }
+      if (LIST) {
+        if (resident) {
+          // four partners per chunk, the next chunk loaded while this one is
+          // computed; padding entries repeat a staged index, masked
+          const int nch = (nlist + 3) >> 2;
+          uint2 ck = nch > 0 ? lp[0] : make_uint2(0u, 0u);
+          for (int k = 0; k < nch; ++k) {
+            const uint2 nx = lp[(size_t)min(k + 1, nch - 1) * NS];
+            const int b4 = 4 * k;
+            pair((int)(ck.x & 0xffffu), true, true);
+            pair((int)(ck.x >> 16), b4 + 1 < nlist, true);
+            pair((int)(ck.y & 0xffffu), b4 + 2 < nlist, true);
+            pair((int)(ck.y >> 16), b4 + 3 < nlist, true);
+            ck = nx;
+          }
+        } else {
+          // lists are ascending in staged index: take this round's entries
+          const int pend = p0 + CAPT;
+          while (kcur < nlist) {
+            const uint2 w = lp[(size_t)(kcur >> 2) * NS];
+            const unsigned h2 = (kcur & 2) ? w.y : w.x;
+            const int j = (int)((kcur & 1) ? (h2 >> 16) : (h2 & 0xffffu));
+            if (j >= pend) break;
+            pair(j - p0, true, true);
+            ++kcur;
+          }
+        }
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -633,9 +730,9 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
         // repeats j with its contribution masked (it must read a staged
         // element: 0 * dx of an unstaged one could be 0 * NaN)
         for (int j = lo; j < hi; j += 2) {
-          pair(j, p0, true);
+          pair(j, true, false);
           const bool two = j + 1 < hi;
-          pair(two ? j + 1 : j, p0, two);
+          pair(two ? j + 1 : j, two, false);
         }
       }
     }
@@ -695,8 +792,14 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
         if (u > L) u -= 2.0 * (u - L);
         if (u < 0.0) u -= 2.0 * u;
       } else {
-        // Python-style modulo as Cython emits it for doubles
-        double m = fmod(u, L);
+        // Python-style modulo as Cython emits it for doubles: fmod, then + L
+        // for a negative remainder.  A step moves far less than L, so fmod's
+        // exact remainder is u itself on (-L, L) and u - L on [L, 2L) (exact:
+        // L <= u < 2L); the general routine only for anything else.
+        double m;
+        if (u > -L && u < L) m = u;
+        else if (u >= L && u < 2.0 * L) m = u - L;
+        else m = fmod(u, L);
         if (m != 0.0 && ((m < 0.0) != (L < 0.0))) m += L;
         u = m;
       }
@@ -709,12 +812,126 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
       }
       v[k] = u;
     }
-    const int cx = cell_coord(v[0], Rp.cl[0], Rp.lx);
-    const int cyy = cell_coord(v[1], Rp.cl[1], Rp.ly);
-    const int c = Rp.cbase + cyy * Rp.lx + cx;
     A.upos[s] = make_double2(v[0], v[1]);
-    A.ucell[s] = c;
-    A.urank[s] = atomicAdd(&A.count[c], 1);
+    if (LIST) {
+      // same slot; request a rebuild once the particle has moved more than
+      // skin/2 (minimum image on periodic axes) from its position at the last one
+      const double2 r0 = A.xref[s];
+      double ex = v[0] - r0.x, ey = v[1] - r0.y;
+      if (Rp.px && fabs(ex) > 0.5 * Rp.L[0]) ex -= copysign(Rp.L[0], ex);
+      if (Rp.py && fabs(ey) > 0.5 * Rp.L[1]) ey -= copysign(Rp.L[1], ey);
+      if (ex * ex + ey * ey > A.hskin2) A.status->rebuild = 1;
+    } else {
+      const int cx = cell_coord(v[0], Rp.cl[0], Rp.lx);
+      const int cyy = cell_coord(v[1], Rp.cl[1], Rp.ly);
+      const int c = Rp.cbase + cyy * Rp.lx + cx;
+      A.ucell[s] = c;
+      A.urank[s] = atomicAdd(&A.count[c], 1);
+    }
+  }
+}
+
+// List build (list mode): for every slot, the staged indices of the partners
+// within (largest pair cutoff + skin) in FP64, ascending, excluding itself and
+// its molecule; the position is recorded as the reference for the skin test.
+// Runs only when a rebuild was requested; the last workgroup to finish clears
+// the request.  Grid-stride over tiles, so that an idle launch is short.
+template <bool MOL, int R>
+__global__ __launch_bounds__(64 * R) void k_build(const StepArgs A) {
+  if (!A.status->rebuild) return;
+  extern __shared__ __align__(16) char smem[];
+  const int CAPT = A.cap;
+  const Staging G = staging(smem, CAPT);
+  double2* s_pos = G.pos;
+  int* s_ty = G.ty;
+  int* s_mol = G.mol;
+  __shared__ TileSh<R> sh;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const size_t NS = (size_t)A.n_total;
+  const int K = A.K;
+  for (int tile = blockIdx.x; tile < A.n_tiles; tile += gridDim.x) {
+    __syncthreads();
+    const int4 T = A.tiles[tile];
+    const DRep& Rp = A.rep[T.x & 0xffffff];
+    const int nrows = (T.x >> 24) + 1;
+    const int cx0 = T.z, W = T.w;
+    const int NR = nrows + 2;
+    tile_setup<R>(Rp, A.cstart, T.y, cx0, W, nrows, sh);
+    const int total = sh.total;
+    const bool resident = total <= CAPT;
+    if (resident) {
+      tile_stage<R, MOL>(sh, NR, 0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, nullptr);
+      __syncthreads();
+    }
+    const double rl2 = Rp.rl2;
+    for (int b = 0; b < sh.nbatch; ++b) {
+      Home<R> h;
+      double2 xi;
+      home_windows<R>(sh, wv, b, lane, nrows, Rp, cx0, W, A.pos, nullptr, h, xi);
+      const int mi = (MOL && h.valid) ? A.mol[h.s] : 0;
+      uint2* out = A.nl4 + (h.valid ? h.s : 0);
+      int n = 0;
+      bool big = false;  // a staged index beyond 16 bits
+      unsigned ch[4] = {0u, 0u, 0u, 0u};  // the chunk being filled
+      for (int p0 = 0; p0 < total; p0 += CAPT) {
+        if (!resident) {
+          __syncthreads();
+          tile_stage<R, MOL>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp,
+                             nullptr);
+          __syncthreads();
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int lo = max(h.lo[d], p0) - p0;
+          const int hi = min(h.hi[d], p0 + CAPT) - p0;
+          for (int j = lo; j < hi; ++j) {
+            if (j + p0 == h.own) continue;
+            if (MOL && s_mol[j] == mi) continue;
+            const double2 pj = s_pos[j];
+            const double dx = pj.x - xi.x, dy = pj.y - xi.y;
+            if (dx * dx + dy * dy > rl2) continue;
+            const int g = j + p0;
+            if (g >= LIST_OVER) big = true;
+            const int slot = n & 3;
+            ch[0] = slot == 0 ? (unsigned)g : ch[0];
+            ch[1] = slot == 1 ? (unsigned)g : ch[1];
+            ch[2] = slot == 2 ? (unsigned)g : ch[2];
+            ch[3] = slot == 3 ? (unsigned)g : ch[3];
+            if (slot == 3 && n < K)
+              out[(size_t)(n >> 2) * NS] = make_uint2(ch[0] | (ch[1] << 16), ch[2] | (ch[3] << 16));
+            ++n;
+          }
+        }
+      }
+      if (h.valid && (n & 3) && n < K) {
+        // partial last chunk: pad with its first entry
+        const int r = n & 3;
+        const unsigned e1 = r > 1 ? ch[1] : ch[0], e2 = r > 2 ? ch[2] : ch[0];
+        out[(size_t)(n >> 2) * NS] = make_uint2(ch[0] | (e1 << 16), e2 | (ch[0] << 16));
+      }
+      if (h.valid) {
+        const bool over = n > K || big;
+        A.ncnt[h.s] = over ? (unsigned short)LIST_OVER : (unsigned short)n;
+        A.xref[h.s] = xi;
+        if (over) atomicAdd(&A.status->overflow, 1ull);
+      }
+      // list entries of the wave, one atomic per wave
+      int e = (h.valid && n <= K && !big) ? n : 0;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) e += __shfl_xor(e, d, 64);
+      if (lane == 0 && e) atomicAdd(&A.status->entries, (unsigned long long)e);
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    __threadfence();
+    const unsigned k = atomicAdd(&A.status->done, 1u);
+    if (k == gridDim.x - 1) {
+      A.status->done = 0;
+      A.status->rebuild = 0;
+      A.status->rebuilds += 1;
+    }
   }
 }
 
@@ -741,12 +958,12 @@ __global__ __launch_bounds__(64) void k_count(const StepArgs A) {
     for (int b = 0; b < sh.nbatch; ++b) {
       Home<1> h;
       double2 xi;
-      home_windows<1>(sh, 0, b, lane, 1, Rp, cx0, W, A.pos, h, xi);
+      home_windows<1>(sh, 0, b, lane, 1, Rp, cx0, W, A.pos, nullptr, h, xi);
       int mi = (MOL && h.valid) ? A.mol[h.s] : 0;
       int cnt = 0;
       for (int p0 = 0; p0 < total; p0 += CAPT) {
         __syncthreads();
-        tile_stage<1, MOL>(sh, 3, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol);
+        tile_stage<1, MOL>(sh, 3, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, nullptr);
         __syncthreads();
         for (int d = 0; d < 3; ++d) {
           const int lo = max(h.lo[d], p0) - p0;
@@ -819,8 +1036,18 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
     if (need) {
       const double2 xi = A.pos[s];
       const int mi = MOL ? A.mol[s] : 0;
-      const int cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
-      const int cy = cell_coord(xi.y, Rp.cl[1], Rp.ly);
+      // the cell the slot is sorted into (list mode: of the last rebuild; cells
+      // are then at least range + skin wide, so the 3x3 stencil still holds
+      // every particle within the range)
+      int cx, cy;
+      if (A.scell) {
+        const int c = A.scell[s] - Rp.cbase;
+        cy = c / Rp.lx;
+        cx = c - cy * Rp.lx;
+      } else {
+        cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
+        cy = cell_coord(xi.y, Rp.cl[1], Rp.ly);
+      }
       for (int oy = -1; oy <= 1; ++oy) {
         int y = cy + oy;
         if (y < 0 || y >= Rp.ly) {
@@ -934,33 +1161,40 @@ __global__ void k_bonds(int n_total, const DRep* __restrict__ rep, const int* __
 // binning, scan, scatter, output
 // ---------------------------------------------------------------------------
 
+// Cell and rank of every slot (a replica's slots and pids are the same
+// range, so rep_of serves both).  With `cond` (list mode) only when a rebuild
+// is requested; grid-stride, so that an idle launch is short.
 __global__ void k_bin(int n_total, const DRep* __restrict__ rep, const int* __restrict__ rep_of,
-                      const double2* __restrict__ pos, int* __restrict__ ucell,
-                      int* __restrict__ urank, int* __restrict__ count) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_total) return;
-  const DRep& R = rep[rep_of[s]];
-  double2 x = pos[s];
-  int c = R.cbase + cell_coord(x.y, R.cl[1], R.ly) * R.lx + cell_coord(x.x, R.cl[0], R.lx);
-  ucell[s] = c;
-  urank[s] = atomicAdd(&count[c], 1);
+                      const double2* __restrict__ pos,
+                      int* __restrict__ ucell, int* __restrict__ urank, int* __restrict__ count,
+                      const DStatus* cond) {
+  if (cond && !cond->rebuild) return;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_total; s += gridDim.x * blockDim.x) {
+    const DRep& R = rep[rep_of[s]];
+    double2 x = pos[s];
+    int c = R.cbase + cell_coord(x.y, R.cl[1], R.ly) * R.lx + cell_coord(x.x, R.cl[0], R.lx);
+    ucell[s] = c;
+    urank[s] = atomicAdd(&count[c], 1);
+  }
 }
 
 #define SCAN_BLOCK 256
 #define SCAN_ITEMS 8
 
 // One workgroup per replica: cell starts = pbase + exclusive prefix of the
-// histogram; the histogram is cleared for the next step.
+// histogram; the histogram is cleared for the next step.  With `cond` (list
+// mode) the scan runs only when a rebuild is requested; the step counter
+// advances either way.
 __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const DRep* __restrict__ rep,
                                                      int* __restrict__ count,
                                                      int* __restrict__ cstart,
                                                      long long* step, int advance,
-                                                     DStatus* status) {
+                                                     DStatus* status, int cond) {
   __shared__ int s_wsum[SCAN_BLOCK / 64];
   __shared__ int s_total;
   const DRep& R = rep[blockIdx.x];
   const int t = threadIdx.x;
-  const int C = R.lx * R.ly;
+  const int C = (cond && !status->rebuild) ? 0 : R.lx * R.ly;
   const int cb = R.cbase;
   int running = R.pbase;
   for (int base = 0; base < C; base += SCAN_BLOCK * SCAN_ITEMS) {
@@ -1004,7 +1238,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const DRep* __restrict__ re
     running += s_total;
     __syncthreads();
   }
-  if (t == 0 && running != R.pbase + R.n) atomicOr(&status->integrity, 2);
+  if (t == 0 && C > 0 && running != R.pbase + R.n) atomicOr(&status->integrity, 2);
   if (advance && blockIdx.x == 0 && t == 0) *step = *step + 1;
 }
 
@@ -1034,6 +1268,56 @@ __global__ void k_scatter(int n_total, const double2* __restrict__ upos,
   if (nmol) nmol[d] = mol[s];
   if (slot_of) slot_of[p] = d;
 }
+
+// List mode rebuild: counting-sort reorder of the slots into scratch columns
+// (cell of each new slot recorded), then copied back in place, so that the
+// buffers of the captured steps never change roles.
+__global__ void k_sort_list(int n_total, const double2* __restrict__ pos,
+                            const int* __restrict__ type, const int* __restrict__ pid,
+                            const int* __restrict__ mol, const int* __restrict__ ucell,
+                            const int* __restrict__ urank, const int* __restrict__ cstart,
+                            double2* __restrict__ spos, int* __restrict__ stype,
+                            int* __restrict__ spid, int* __restrict__ smol,
+                            int* __restrict__ slot_of, int* __restrict__ scell, int n_cells,
+                            DStatus* status) {
+  if (!status->rebuild) return;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_total; s += gridDim.x * blockDim.x) {
+    const int c = ucell[s];
+    if (c < 0 || c >= n_cells) {
+      atomicOr(&status->integrity, 1);
+      continue;
+    }
+    const int d = cstart[c] + urank[s];
+    if (d < 0 || d >= n_total) {
+      atomicOr(&status->integrity, 1);
+      continue;
+    }
+    const int p = pid[s];
+    spos[d] = pos[s];
+    stype[d] = type[s];
+    spid[d] = p;
+    if (smol) smol[d] = mol[s];
+    if (slot_of) slot_of[p] = d;
+    scell[d] = c;
+  }
+}
+
+__global__ void k_copy_list(int n_total, const double2* __restrict__ spos,
+                            const int* __restrict__ stype, const int* __restrict__ spid,
+                            const int* __restrict__ smol, double2* __restrict__ pos,
+                            int* __restrict__ type, int* __restrict__ pid, int* __restrict__ mol,
+                            const DStatus* status) {
+  if (!status->rebuild) return;
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_total; s += gridDim.x * blockDim.x) {
+    pos[s] = spos[s];
+    type[s] = stype[s];
+    pid[s] = spid[s];
+    if (mol) mol[s] = smol[s];
+  }
+}
+
+// Requests a rebuild (set_state, re-tiling, diagnostics).
+__global__ void k_request(DStatus* status) { status->rebuild = 1; }
 
 __global__ void k_gather_out(int n_total, const double2* __restrict__ pos,
                              const int* __restrict__ type, const int* __restrict__ pid,
@@ -1147,6 +1431,18 @@ struct rds_ctx {
   int cap = 384;        // staged particles per LDS round
   int target = 56;      // particles per tile row strip
   size_t smem_bytes = 0;
+  // list mode (skin > 0)
+  bool list = false;
+  double skin = 0.0;
+  int K = 64;                    // list entries per particle (multiple of 4)
+  uint2* nl4 = nullptr;          // [K/4 + 1][N] chunks (one spare row)
+  unsigned short* ncnt = nullptr;
+  double2* xref = nullptr;
+  int* scell = nullptr;
+  double2* spos = nullptr;       // rebuild scratch columns
+  int* stype = nullptr;
+  int* spid = nullptr;
+  int* smol = nullptr;
 };
 
 namespace {
@@ -1182,31 +1478,43 @@ StepArgs make_args(rds_ctx* c, const Buf& cur) {
   a.sblocks = c->d_sblocks;
   a.include_rf = c->include_rf;
   a.status = c->d_status;
+  a.nl4 = c->nl4;
+  a.ncnt = c->ncnt;
+  a.K = c->K;
+  a.xref = c->xref;
+  a.scell = c->list ? c->scell : nullptr;
+  a.hskin2 = 0.25 * c->skin * c->skin;
   return a;
 }
 
-template <typename Real, bool MOL, int KM>
+template <typename Real, bool MOL, int KM, bool LIST>
 void launch_force_r(rds_ctx* c, const StepArgs& a, int mode) {
   dim3 g((unsigned)c->n_tiles);
   const size_t sm = c->smem_bytes;
   switch (c->rows) {
-    case 4: hipLaunchKernelGGL((k_force<Real, MOL, KM, 4>), g, dim3(256), sm, c->stream, a, mode, c->pw); break;
-    case 2: hipLaunchKernelGGL((k_force<Real, MOL, KM, 2>), g, dim3(128), sm, c->stream, a, mode, c->pw); break;
-    default: hipLaunchKernelGGL((k_force<Real, MOL, KM, 1>), g, dim3(64), sm, c->stream, a, mode, c->pw); break;
+    case 4: hipLaunchKernelGGL((k_force<Real, MOL, KM, 4, LIST>), g, dim3(256), sm, c->stream, a, mode, c->pw); break;
+    case 2: hipLaunchKernelGGL((k_force<Real, MOL, KM, 2, LIST>), g, dim3(128), sm, c->stream, a, mode, c->pw); break;
+    default: hipLaunchKernelGGL((k_force<Real, MOL, KM, 1, LIST>), g, dim3(64), sm, c->stream, a, mode, c->pw); break;
   }
 }
 
-template <typename Real, bool MOL>
+template <typename Real, bool MOL, bool LIST>
 void launch_force_m(rds_ctx* c, const StepArgs& a, int mode) {
-  if (!c->fast) launch_force_r<Real, MOL, 0>(c, a, mode);
-  else if (c->pw == 3) launch_force_r<Real, MOL, 2>(c, a, mode);
-  else launch_force_r<Real, MOL, 1>(c, a, mode);
+  if (!c->fast) launch_force_r<Real, MOL, 0, LIST>(c, a, mode);
+  else if (c->pw == 3) launch_force_r<Real, MOL, 2, LIST>(c, a, mode);
+  else launch_force_r<Real, MOL, 1, LIST>(c, a, mode);
+}
+
+template <typename Real, bool LIST>
+void launch_force_l(rds_ctx* c, const StepArgs& a, int mode) {
+  if (c->has_mol) launch_force_m<Real, true, LIST>(c, a, mode);
+  else launch_force_m<Real, false, LIST>(c, a, mode);
 }
 
 template <typename Real>
 void launch_force_p(rds_ctx* c, const StepArgs& a, int mode) {
-  if (c->has_mol) launch_force_m<Real, true>(c, a, mode);
-  else launch_force_m<Real, false>(c, a, mode);
+  if (c->list) launch_force_l<Real, true>(c, a, mode);
+  else launch_force_l<Real, false>(c, a, mode);
 }
 
 void launch_force(rds_ctx* c, const StepArgs& a, int mode) {
@@ -1215,9 +1523,67 @@ void launch_force(rds_ctx* c, const StepArgs& a, int mode) {
 }
 
 inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+// grid of the conditional grid-stride kernels of list mode
+inline unsigned nblk_gs(int64_t n) { return std::min(nblk(n, 256), 4096u); }
 
 // kernel ids for timing
-enum { KT_FORCE = 0, KT_SCAN = 1, KT_SCATTER = 2, KT_REACT = 3, KT_BONDS = 4, KT_N = 5 };
+enum { KT_FORCE = 0, KT_SCAN = 1, KT_SCATTER = 2, KT_REACT = 3, KT_BONDS = 4, KT_LIST = 5, KT_N = 6 };
+
+template <bool MOL>
+void launch_build(rds_ctx* c, const StepArgs& a) {
+  const unsigned g = (unsigned)std::min<int64_t>(c->n_tiles, 4096);
+  const size_t sm = c->smem_bytes;
+  switch (c->rows) {
+    case 4: hipLaunchKernelGGL((k_build<MOL, 4>), dim3(g), dim3(256), sm, c->stream, a); break;
+    case 2: hipLaunchKernelGGL((k_build<MOL, 2>), dim3(g), dim3(128), sm, c->stream, a); break;
+    default: hipLaunchKernelGGL((k_build<MOL, 1>), dim3(g), dim3(64), sm, c->stream, a); break;
+  }
+}
+
+// List mode rebuild, each kernel a no-op unless a rebuild is requested: bin
+// `pos`, scan (advancing the step counter when `advance`), sort pos / type /
+// pid / molecule into scratch and back, build the lists.  `mark(kernel id,
+// begin)` brackets the launches for timing.
+template <typename Mark>
+void enqueue_rebuild(rds_ctx* c, double2* pos, int* type, int advance, Mark&& mark) {
+  const int N = (int)c->n_total;
+  int* pid = c->buf[0].pid;
+  int* mol = c->has_mol ? c->buf[0].mol : nullptr;
+  mark(KT_LIST, true);
+  hipLaunchKernelGGL(k_bin, dim3(nblk_gs(N)), dim3(256), 0, c->stream, N, c->d_rep, c->d_rep_of,
+                     pos, c->ucell, c->urank, c->count, c->d_status);
+  mark(KT_LIST, false);
+  mark(KT_SCAN, true);
+  hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
+                     c->cstart, c->d_step, advance, c->d_status, 1);
+  mark(KT_SCAN, false);
+  mark(KT_SCATTER, true);
+  hipLaunchKernelGGL(k_sort_list, dim3(nblk_gs(N)), dim3(256), 0, c->stream, N, pos, type, pid,
+                     mol, c->ucell, c->urank, c->cstart, c->spos, c->stype, c->spid,
+                     c->has_mol ? c->smol : nullptr, c->any_bonds ? c->slot_of : nullptr,
+                     c->scell, (int)c->n_cells, c->d_status);
+  hipLaunchKernelGGL(k_copy_list, dim3(nblk_gs(N)), dim3(256), 0, c->stream, N, c->spos,
+                     c->stype, c->spid, c->has_mol ? c->smol : nullptr, pos, type, pid, mol,
+                     c->d_status);
+  mark(KT_SCATTER, false);
+  Buf b;
+  b.pos = pos;
+  b.type = type;
+  b.pid = pid;
+  b.mol = mol;
+  StepArgs a = make_args(c, b);
+  mark(KT_LIST, true);
+  if (c->has_mol) launch_build<true>(c, a);
+  else launch_build<false>(c, a);
+  mark(KT_LIST, false);
+}
+
+// A rebuild outside the step loop (state loaded, re-tiled, diagnostics).
+void force_rebuild(rds_ctx* c) {
+  Buf& cur = c->buf[c->parity];
+  hipLaunchKernelGGL(k_request, dim3(1), dim3(1), 0, c->stream, c->d_status);
+  enqueue_rebuild(c, cur.pos, cur.type, 0, [](int, bool) {});
+}
 
 struct Timer {
   hipEvent_t ev[2 * KT_N * 2];
@@ -1249,12 +1615,35 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
     mark(KT_BONDS, false);
   }
   StepArgs a = make_args(c, cur);
+  if (c->list) {
+    // positions move in place (same slots) into the other buffer; types and
+    // the sort are touched only by a rebuild; reactions write the other type
+    // column (without reactions both buffers share one type column)
+    a.upos = nxt.pos;
+    mark(KT_FORCE, true);
+    launch_force(c, a, MODE_STEP);
+    mark(KT_FORCE, false);
+    enqueue_rebuild(c, nxt.pos, cur.type, 1, mark);
+    if (c->any_rx) {
+      StepArgs r = make_args(c, nxt);
+      r.type = cur.type;
+      r.type_out = nxt.type;
+      mark(KT_REACT, true);
+      if (c->has_mol)
+        hipLaunchKernelGGL((k_react<true>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
+      else
+        hipLaunchKernelGGL((k_react<false>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
+      mark(KT_REACT, false);
+    }
+    c->parity ^= 1;
+    return;
+  }
   mark(KT_FORCE, true);
   launch_force(c, a, MODE_STEP);
   mark(KT_FORCE, false);
   mark(KT_SCAN, true);
   hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
-                     c->cstart, c->d_step, 1, c->d_status);
+                     c->cstart, c->d_step, 1, c->d_status, 0);
   mark(KT_SCAN, false);
   mark(KT_SCATTER, true);
   hipLaunchKernelGGL(k_scatter, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
@@ -1474,6 +1863,11 @@ int rds_set_tiling(rds_ctx* c, int32_t tile_rows, int32_t tile_particles) {
   if (!build_tiles(c, tile_rows, tile_particles)) return RDS_E_ARG;
   RDS_TRY(upload_tiles(c));
   c->target = tile_particles > 0 ? tile_particles : 56;
+  if (c->loaded && c->list) {
+    // lists hold staged indices of the old tiles
+    force_rebuild(c);
+    HIP_TRY(hipGetLastError());
+  }
   if (c->loaded) RDS_TRY(build_graph(c));
   return RDS_OK;
 }
@@ -1492,16 +1886,20 @@ int rds_ctx_destroy(rds_ctx* c) {
   if (!c) return RDS_OK;
   free_graph(c);
   if (c->stream) hipStreamSynchronize(c->stream);
-  void* ptrs[] = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->d_sblocks, c->upos, c->ucell, c->urank,
-                  c->count, c->cstart, c->d_step, c->d_status, c->boff, c->bpart, c->br0,
-                  c->bu0, c->fbond, c->slot_of};
-  for (void* p : ptrs)
-    if (p) hipFree(p);
+  std::vector<void*> ptrs = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->d_sblocks,
+                             c->upos, c->ucell, c->urank, c->count, c->cstart, c->d_step,
+                             c->d_status, c->boff, c->bpart, c->br0, c->bu0, c->fbond,
+                             c->slot_of, c->nl4, c->ncnt, c->xref, c->scell, c->spos,
+                             c->stype, c->spid, c->smol};
   for (int k = 0; k < 2; ++k) {
     void* b[] = {c->buf[k].pos, c->buf[k].type, c->buf[k].type0, c->buf[k].pid, c->buf[k].mol};
-    for (void* p : b)
-      if (p) hipFree(p);
+    for (void* p : b) ptrs.push_back(p);
   }
+  // list mode shares some columns between the two buffers: free each once
+  std::sort(ptrs.begin(), ptrs.end());
+  ptrs.erase(std::unique(ptrs.begin(), ptrs.end()), ptrs.end());
+  for (void* p : ptrs)
+    if (p) hipFree(p);
   if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
   delete c;
   return RDS_OK;
@@ -1537,6 +1935,13 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   const int spc = cfg->cells_per_cutoff != 0 ? std::abs(cfg->cells_per_cutoff) : 1;
   if (spc != 1) return bail(fail(RDS_E_ARG, "only cells_per_cutoff = 1 (3x3 stencil) is built"));
   const double target = cfg->tile_particles > 0 ? cfg->tile_particles : 56;
+  if (!(cfg->skin >= 0.0) || !std::isfinite(cfg->skin))
+    return bail(fail(RDS_E_ARG, "skin must be finite and >= 0"));
+  c->skin = cfg->skin;
+  c->list = cfg->skin > 0.0;
+  c->K = cfg->list_capacity > 0 ? cfg->list_capacity : 64;
+  if (c->K > 4096) return bail(fail(RDS_E_ARG, "list_capacity %d above 4096", c->K));
+  c->K = (c->K + 3) / 4 * 4;  // whole chunks of four entries
 
   std::vector<double> rcut(cfg->n_models);
   c->h_models.resize(cfg->n_models);
@@ -1572,6 +1977,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     if (s.n_particles < 1) return bail(fail(RDS_E_ARG, "replica %d: no particles", r));
     double rcm = rcut[s.model];
     if (!(rcm > 0.0)) rcm = std::min(s.L[0], s.L[1]);  // no pair range: one cell suffices
+    else rcm += c->skin;                                // list mode: cells >= range + skin
     int lcell[2];
     for (int k = 0; k < 2; ++k) {
       if (!(s.L[k] > 0.0)) return bail(fail(RDS_E_ARG, "replica %d: box length must be > 0", r));
@@ -1583,8 +1989,8 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
         if (rcut[s.model] > 0.0)
           return bail(fail(RDS_E_ARG,
                            "replica %d: periodic box length %g holds fewer than 3 cells of the "
-                           "interaction range %g",
-                           r, s.L[k], rcut[s.model]));
+                           "interaction range %g (+ skin %g)",
+                           r, s.L[k], rcut[s.model], c->skin));
       }
       lcell[k] = l;
       d.L[k] = s.L[k];
@@ -1610,6 +2016,10 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     }
     for (int q = 0; q < MAXRX; ++q) d.rate[q] = s.rate[q];
     d.dt = s.dt;
+    {
+      const double rl = std::sqrt(c->h_models[s.model].rc2max) + c->skin;
+      d.rl2 = rl * rl;
+    }
     pbase += s.n_particles;
     cbase += (int64_t)d.lx * d.ly;
     if (pbase > 0x7fffffff || cbase > 0x7ffffff0)
@@ -1621,19 +2031,48 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
 
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   const size_t N = (size_t)c->n_total, C = (size_t)c->n_cells;
-#define A_(p, n) \
-  if ((rc = dalloc(&(p), (n))) != RDS_OK) return bail(rc)
+#define A_(p, n)                                          \
+  do {                                                    \
+    if ((rc = dalloc(&(p), (n))) != RDS_OK) return bail(rc); \
+  } while (0)
   A_(c->d_rep, c->n_rep);
   A_(c->d_models, c->n_models);
   A_(c->d_rep_of, N);
-  for (int k = 0; k < 2; ++k) {
-    A_(c->buf[k].pos, N);
-    A_(c->buf[k].type, N);
-    A_(c->buf[k].type0, N);
-    A_(c->buf[k].pid, N);
-    if (c->has_mol) A_(c->buf[k].mol, N);
+  if (!c->list) {
+    for (int k = 0; k < 2; ++k) {
+      A_(c->buf[k].pos, N);
+      A_(c->buf[k].type, N);
+      A_(c->buf[k].type0, N);
+      A_(c->buf[k].pid, N);
+      if (c->has_mol) A_(c->buf[k].mol, N);
+    }
+    A_(c->upos, N);
+  } else {
+    // positions ping-pong; pid / molecule columns are shared (only a rebuild
+    // reorders them, in place); types ping-pong only when reactions write them
+    A_(c->buf[0].pos, N);
+    A_(c->buf[1].pos, N);
+    A_(c->buf[0].type, N);
+    if (c->any_rx) {
+      A_(c->buf[1].type, N);
+    } else {
+      c->buf[1].type = c->buf[0].type;
+    }
+    A_(c->buf[0].pid, N);
+    c->buf[1].pid = c->buf[0].pid;
+    if (c->has_mol) {
+      A_(c->buf[0].mol, N);
+      c->buf[1].mol = c->buf[0].mol;
+      A_(c->smol, N);
+    }
+    A_(c->nl4, (size_t)(c->K / 4 + 1) * N);
+    A_(c->ncnt, N);
+    A_(c->xref, N);
+    A_(c->scell, N);
+    A_(c->spos, N);
+    A_(c->stype, N);
+    A_(c->spid, N);
   }
-  A_(c->upos, N);
   A_(c->ucell, N);
   A_(c->urank, N);
   A_(c->count, C);
@@ -1718,18 +2157,23 @@ int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32
   hipLaunchKernelGGL(k_load, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, (const double2*)x,
                      types, c->has_mol ? mol : nullptr, a.pos, a.type, a.pid,
                      c->has_mol ? a.mol : nullptr);
-  hipLaunchKernelGGL(k_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep, c->d_rep_of,
-                     a.pos, c->ucell, c->urank, c->count);
   DStatus st;
   memset(&st, 0, sizeof st);
   st.err_key = ~0ull;
   HIP_TRY(hipMemcpyAsync(c->d_status, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
-                     c->cstart, c->d_step, 0, c->d_status);
-  hipLaunchKernelGGL(k_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos, c->ucell,
-                     c->urank, c->cstart, a.type, a.pid, a.mol, b.pos, b.type, b.pid, b.mol,
-                     c->slot_of, (int)c->n_cells, c->d_status);
-  c->parity = 1;
+  if (c->list) {
+    // sorted in place, lists built
+    force_rebuild(c);
+  } else {
+    hipLaunchKernelGGL(k_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep,
+                       c->d_rep_of, a.pos, c->ucell, c->urank, c->count, nullptr);
+    hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep,
+                       c->count, c->cstart, c->d_step, 0, c->d_status, 0);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos,
+                       c->ucell, c->urank, c->cstart, a.type, a.pid, a.mol, b.pos, b.type, b.pid,
+                       b.mol, c->slot_of, (int)c->n_cells, c->d_status);
+    c->parity = 1;
+  }
   long long s0 = step0;
   HIP_TRY(hipMemcpyAsync(c->d_step, &s0, sizeof s0, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1802,6 +2246,9 @@ int rds_sync(rds_ctx* c, rds_status* out) {
   s.stream_error = st.stream;
   s.reaction_events = (int64_t)st.events;
   s.integrity = st.integrity;
+  s.rebuilds = (int64_t)st.rebuilds;
+  s.list_overflow = (int64_t)st.overflow;
+  s.list_entries = (int64_t)st.entries;
   if (st.bounds) {
     s.error_step = (int64_t)(st.err_key >> 32);
     unsigned lo = (unsigned)(st.err_key & 0xffffffffu);
@@ -1870,6 +2317,8 @@ int rds_neighbor_counts(rds_ctx* c, double r, int32_t* counts) {
     if (r > std::min(d.cl[0], d.cl[1]))
       return fail(RDS_E_ARG, "probe radius %g exceeds the cell length of replica %d", r, q);
   }
+  // list mode: cells are those of the last rebuild; re-sort for the current positions
+  if (c->list) force_rebuild(c);
   Buf& cur = c->buf[c->parity];
   StepArgs a = make_args(c, cur);
   a.iout = counts;

@@ -125,11 +125,18 @@ def make_replica(model_index, n_particles, L_box, periodic, dt, tau_by_tauD, rat
 
 
 class Engine:
-    """One engine context over one or more replicas on one GPU."""
+    """One engine context over one or more replicas on one GPU.
+
+    skin = 0 re-sorts by cell and scans the 3x3 stencil every step; skin > 0
+    keeps Verlet lists within (pair range + skin), rebuilt with the sort only
+    after some particle moved more than skin/2 (list_capacity entries per
+    particle, 0 = 64; a particle with more falls back to the stencil scan).
+    """
 
     def __init__(self, models, replicas, precision='mixed', rng='philox',
                  reaction_prob='linear', bonds=None, has_molecules=False,
-                 tile_particles=0, graph_steps=0, tile_rows=1, device='cuda'):
+                 tile_particles=0, graph_steps=0, tile_rows=1, device='cuda',
+                 skin=0.0, list_capacity=0):
         self.lib = L.load()
         if not torch.cuda.is_available():
             raise RuntimeError('the engine needs a ROCm GPU; none is visible')
@@ -154,6 +161,9 @@ class Engine:
         cfg.cells_per_cutoff = 1
         cfg.graph_steps = int(graph_steps)
         cfg.tile_rows = int(tile_rows)
+        cfg.skin = float(skin)
+        cfg.list_capacity = int(list_capacity)
+        self.skin = float(skin)
         self._bond_arrays = None
         if bonds is not None and len(bonds[0]) > 0:
             bi = np.ascontiguousarray(bonds[0], dtype=np.int32)

@@ -27,7 +27,8 @@ def rates_of(model):
 
 
 def engine_for(s, precision='fp64', rng='philox', seed=2001, graph_steps=0,
-               reaction_prob='linear', tile_particles=0, tile_rows=1):
+               reaction_prob='linear', tile_particles=0, tile_rows=1, skin=0.0,
+               list_capacity=0):
     from pyrdsim_amd.engine import Engine, build_model, make_replica
     m = s['model']
     model, rates = build_model(m['interactions'], m['reactions'], len(m['tau_by_tauD']))
@@ -36,7 +37,8 @@ def engine_for(s, precision='fp64', rng='philox', seed=2001, graph_steps=0,
     has_mol = s['bonds'] is not None
     eng = Engine([model], [rep], precision=precision, rng=rng, reaction_prob=reaction_prob,
                  bonds=s['bonds'], has_molecules=has_mol, graph_steps=graph_steps,
-                 tile_particles=tile_particles, tile_rows=tile_rows)
+                 tile_particles=tile_particles, tile_rows=tile_rows, skin=skin,
+                 list_capacity=list_capacity)
     eng.set_state(s['x'], s['types'], s['molecule_ids'][:, 0] if has_mol else None)
     return eng
 
