@@ -1,8 +1,9 @@
 """Autotuner for the force kernel's work decomposition.
 
-The tile shape -- cell rows per tile (one wavefront per row) and particles per
-row strip -- trades LDS staging overhead (bigger tiles stage fewer halo cells
-per particle) against parallelism, LDS occupancy and lane utilisation.  The
+The tile shape -- cell rows per tile (one wavefront per row, the tile's home
+particles dealt to the wavefronts 64 at a time) and particles per row strip --
+trades LDS staging overhead (bigger tiles stage fewer halo cells per particle)
+against parallelism, LDS occupancy and lane utilisation.  The
 best shape depends on the workload (density, replica sizes, GPU fill), so it is
 measured: each candidate runs a few real steps on the live state and the
 force kernel's HIP-event time decides.  Results are cached by a workload
@@ -12,7 +13,7 @@ signature in a small JSON file.
 import json
 import os
 
-CANDIDATES = ((1, 40), (1, 56), (2, 40), (2, 56), (2, 80), (2, 112), (4, 40), (4, 56), (4, 96),
+CANDIDATES = ((1, 56), (2, 48), (2, 56), (2, 64), (2, 80), (4, 48), (4, 56), (4, 64), (4, 96),
               (4, 144))
 
 

@@ -227,6 +227,13 @@ __device__ __forceinline__ void philox_uniforms(const DRep& R, int pid, long lon
 // Cell coordinate floor(x / cl) (the reference's x // cl, pyrdsim.pyx:1497),
 // clamped into [0, l): a periodic wrap may return exactly L, and a coordinate
 // that left the box (reported separately) must still index a valid cell.
+// The reference's minimum image dx - L*round(dx/L) (pyrdsim.pyx:1628-1640).
+// For |dx| < L/4 the rounded quotient is +-0 and the result is dx (up to the
+// sign of a zero, which no distance sees), so the division is skipped there.
+__device__ __forceinline__ double min_image(double dx, double L) {
+  return fabs(dx) < 0.25 * L ? dx : dx - L * round(dx / L);
+}
+
 __device__ __forceinline__ int cell_coord(double x, double cl, int l) {
   double q = floor(x / cl);
   if (!(q >= 0.0)) return 0;
@@ -264,6 +271,7 @@ struct TileSh {
   int seg_slot[R + 2][3], seg_n[R + 2][3], seg_off[R + 2][3];
   double seg_sx[R + 2][3];
   double row_sy[R + 2];
+  int hp[R + 1];             // home particles of the home rows before row q
   int total, nbatch;
 };
 
@@ -345,10 +353,14 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
     }
     sh.row_base[NR] = base;
     sh.total = base;
+    int hsum = 0;
     for (int q = 0; q < nrows; ++q) {
-      int nh = sh.off[q + 1][W + 1] - sh.off[q + 1][1];
-      nb = max(nb, (nh + 63) >> 6);
+      sh.hp[q] = hsum;
+      hsum += sh.off[q + 1][W + 1] - sh.off[q + 1][1];
     }
+    sh.hp[nrows] = hsum;
+    // the home particles of all rows are dealt to the R waves, 64 per wave
+    nb = (hsum + 64 * R - 1) / (64 * R);
     sh.nbatch = nb;
   }
   __syncthreads();
@@ -405,9 +417,11 @@ __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
   }
 }
 
-// Home particle of lane `lane` of wave q in batch b, and its three windows.
-// The home cell is the one the slot was sorted into: scell (list mode, cells of
-// the last rebuild) or, with scell null, the cell of the current position.
+// Home particle of lane `lane` of wave wv in batch b, and its three windows.
+// The home particles of the tile's rows, concatenated, are dealt to the waves
+// 64 at a time: home number (b * R + wv) * 64 + lane.  The home cell is the one
+// the slot was sorted into: scell (list mode, cells of the last rebuild) or,
+// with scell null, the cell of the current position.
 template <int R>
 struct Home {
   int s;
@@ -417,17 +431,18 @@ struct Home {
 };
 
 template <int R>
-__device__ __forceinline__ void home_windows(const TileSh<R>& sh, int q, int b, int lane,
+__device__ __forceinline__ void home_windows(const TileSh<R>& sh, int wv, int b, int lane,
                                              int nrows, const DRep& Rp, int cx0, int W,
                                              const double2* __restrict__ pos,
                                              const int* __restrict__ scell, Home<R>& h,
                                              double2& xi) {
+  const int g = (b * R + wv) * 64 + lane;
+  int q = 0;  // home row of this lane
+#pragma unroll
+  for (int r = 1; r < R; ++r) q = (r < nrows && g >= sh.hp[r]) ? r : q;
   const int hr = q + 1;  // staged row of the home row
-  const bool rowok = q < nrows;
-  const int hb = rowok ? sh.cs[hr][1] : 0;
-  const int he = rowok ? sh.cs[hr][1] + (sh.off[hr][W + 1] - sh.off[hr][1]) : 0;
-  h.s = hb + b * 64 + lane;
-  h.valid = rowok && h.s < he;
+  h.valid = g < sh.hp[nrows];
+  h.s = h.valid ? sh.cs[hr][1] + (g - sh.hp[q]) : 0;
   xi = make_double2(0.0, 0.0);
   h.w = 0;
   h.own = -1;
@@ -1067,8 +1082,8 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
             if (MOL && A.mol[j] == mi) continue;
             const double2 xj = A.pos[j];
             double dx = xj.x - xi.x, dy = xj.y - xi.y;
-            if (Rp.px) dx = dx - Rp.L[0] * round(dx / Rp.L[0]);
-            if (Rp.py) dy = dy - Rp.L[1] * round(dy / Rp.L[1]);
+            if (Rp.px) dx = min_image(dx, Rp.L[0]);
+            if (Rp.py) dy = min_image(dy, Rp.L[1]);
             const double d = sqrt(dx * dx + dy * dy);
             const int tj = A.type[j];
 #pragma unroll
@@ -1147,8 +1162,8 @@ __global__ void k_bonds(int n_total, const DRep* __restrict__ rep, const int* __
   for (int b = boff[p]; b < boff[p + 1]; ++b) {
     double2 xj = pos[slot_of[bpart[b]]];
     double dx = xj.x - xi.x, dy = xj.y - xi.y;
-    if (R.px) dx = dx - R.L[0] * round(dx / R.L[0]);
-    if (R.py) dy = dy - R.L[1] * round(dy / R.L[1]);
+    if (R.px) dx = min_image(dx, R.L[0]);
+    if (R.py) dy = min_image(dy, R.L[1]);
     double d = sqrt(dx * dx + dy * dy);
     double f = bu0[b] * (d - br0[b]);
     fx += f * (dx / d);
