@@ -107,10 +107,15 @@ def time_engine(eng, steps, warmup):
     return time.perf_counter() - t0
 
 
-def accuracy_leg(plan, states, models, reps, x_mixed, t_mixed, steps, n_check):
-    """RDF and activated fraction of the batched FP32-force run against the
-    FP64 path: a stratified subset of replicas is re-run alone in FP64 for the
-    same number of steps with the same Philox streams, from the same states."""
+def accuracy_leg(plan, states, models, reps, x0, t0s, graph_steps, tiling, steps, n_check,
+                 chunk=1000):
+    """Statistical accuracy of the batched FP32-force runs (mixed: FP64
+    positions; compensated: float-pair positions) against the FP64 path over
+    `steps` steps with the same Philox streams from the same states: a
+    stratified subset of replicas is re-run alone in FP64.  The RDF and the
+    activated fraction are averaged over snapshots every `chunk` steps in the
+    second half of the run; the largest position deviation is taken after the
+    first chunk (later the trajectories decorrelate, as chaotic dynamics do)."""
     from pyrdsim_amd import analysis
     from pyrdsim_amd.engine import Engine
     off = np.concatenate([[0], np.cumsum([p['n'] for p in plan])])
@@ -120,24 +125,105 @@ def accuracy_leg(plan, states, models, reps, x_mixed, t_mixed, steps, n_check):
                      key=lambda r: plan[r]['n'])
         m = max(1, n_check // 2)
         picks += [idx[int((i + 0.5) * len(idx) / m)] for i in range(m)]
-    rdf_err, act_err = [], []
+    n_chunks = max(2, steps // chunk)
+    second = range(n_chunks // 2, n_chunks)
+    geo = {r: ([reps[r].L[0], reps[r].L[1]], [reps[r].periodic[0], reps[r].periodic[1]])
+           for r in picks}
+
+    def observe(acc, r, x, t, k):
+        L, per = geo[r]
+        if k == 0:
+            acc.setdefault('x1', {})[r] = x.copy()
+        if k in second:
+            _, g = analysis.rdf(x, L, per)
+            acc.setdefault('g', {}).setdefault(r, []).append(g)
+            acc.setdefault('a', {}).setdefault(r, []).append(analysis.activated_fraction(t))
+
+    ref = {}
     for r in picks:
-        p = plan[r]
-        e = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64')
+        e = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64',
+                   graph_steps=graph_steps)
         e.set_state(states[r][0], states[r][1])
-        e.run(steps)
-        x64, t64 = e.state()
+        for k in range(n_chunks):
+            e.run(chunk)
+            x, t = e.state()
+            observe(ref, r, x, t, k)
         e.close()
-        L = [reps[r].L[0], reps[r].L[1]]
-        per = [reps[r].periodic[0], reps[r].periodic[1]]
-        xm = x_mixed[off[r]:off[r + 1]]
-        tm = t_mixed[off[r]:off[r + 1]]
-        _, g64 = analysis.rdf(x64, L, per)
-        _, gm = analysis.rdf(xm, L, per)
-        rdf_err.append(analysis.rdf_error(gm, g64))
-        act_err.append(abs(analysis.activated_fraction(tm) - analysis.activated_fraction(t64)))
-    return dict(replicas=len(picks), steps=int(steps), reference='fp64 single replica, same streams',
-                rdf_l1_rel_max=float(max(rdf_err)), activated_fraction_abs_max=float(max(act_err)))
+    out = dict(replicas=len(picks), steps=int(n_chunks * chunk), snapshots=len(second),
+               reference='fp64 single replica, same streams')
+    for prec in ('mixed', 'compensated'):
+        e = Engine(models, reps, precision=prec, graph_steps=graph_steps, tile_rows=tiling[0],
+                   tile_particles=tiling[1])
+        e.set_state(x0, t0s)
+        acc = {}
+        for k in range(n_chunks):
+            e.run(chunk)
+            xa, ta = e.state()
+            for r in picks:
+                observe(acc, r, xa[off[r]:off[r + 1]], ta[off[r]:off[r + 1]], k)
+        e.close()
+        rdf_err, act_err, dev = [], [], []
+        for r in picks:
+            L, per = geo[r]
+            rdf_err.append(analysis.rdf_error(np.mean(acc['g'][r], axis=0),
+                                              np.mean(ref['g'][r], axis=0)))
+            act_err.append(abs(np.mean(acc['a'][r]) - np.mean(ref['a'][r])))
+            d = acc['x1'][r] - ref['x1'][r]
+            for q in range(2):
+                if per[q] > 0:
+                    d[:, q] -= L[q] * np.round(d[:, q] / L[q])
+            dev.append(float(np.abs(d).max()))
+        out[prec] = {'rdf_l1_rel_max': float(max(rdf_err)),
+                     'activated_fraction_abs_max': float(max(act_err)),
+                     'position_dev_max_after_%d_steps' % chunk: float(max(dev))}
+    out['rdf_l1_rel_max'] = out['mixed']['rdf_l1_rel_max']
+    out['activated_fraction_abs_max'] = out['mixed']['activated_fraction_abs_max']
+    return out
+
+
+def measure_traffic(cache, rows, per, replicas, timeout=300):
+    """HBM bytes per force-kernel launch from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE) over a few eager steps of this workload in a
+    child process, each scaled by what the same counter reads for an
+    elementwise kernel that reads and writes 256 MiB (--copy-calibration).
+    Run before this process touches the GPU.  None when rocprofv3 is missing
+    or a pass fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which('rocprofv3')
+    if not prof:
+        return None
+    tmp = tempfile.mkdtemp(prefix='rds_traffic_')
+    child = [sys.executable, os.path.abspath(__file__), '--steps', '6', '--warmup', '2',
+             '--graph-steps', '0', '--replicas', str(replicas), '--no-plain', '--no-cpu',
+             '--no-accuracy', '--no-autotune', '--no-traffic', '--tile-rows', str(rows),
+             '--tile-particles', str(per), '--copy-calibration', '--ic-cache', cache]
+    out = {}
+    try:
+        for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
+            d = os.path.join(tmp, counter)
+            cmd = [prof, '--pmc', counter, '--output-format', 'csv', '-d', d, '-o', 'c', '--'] + child
+            subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           timeout=timeout)
+            files = glob.glob(os.path.join(d, '**', 'c_counter_collection.csv'), recursive=True)
+            if not files:
+                return None
+            with open(files[0]) as f:
+                recs = [r for r in csv.DictReader(f) if r['Counter_Name'] == counter]
+            force = [float(r['Counter_Value']) for r in recs if 'k_force' in r['Kernel_Name']]
+            cal = [float(r['Counter_Value']) for r in recs if 'AUnaryFunctor' in r['Kernel_Name']]
+            if not force or not cal or cal[0] <= 0:
+                return None
+            factor = cal[0] / (268435456 / 1024)       # KiB counted per KiB moved
+            out[counter] = float(np.mean(force)) * 1024 / factor
+    except (subprocess.SubprocessError, OSError, KeyError, ValueError):
+        return None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return out
 
 
 def cpu_baseline(plan, states, budget_s):
@@ -195,10 +281,13 @@ def main():
     ap.add_argument('--no-autotune', dest='autotune', action='store_false')
     ap.add_argument('--no-accuracy', action='store_true')
     ap.add_argument('--accuracy-replicas', type=int, default=8)
+    ap.add_argument('--accuracy-steps', type=int, default=10000)
     ap.add_argument('--copy-calibration', action='store_true',
                     help='run one elementwise kernel of known traffic (256 MiB read + 256 MiB '
                          'written) first, to scale HBM counters in a profile of this command')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-traffic', dest='traffic', action='store_false',
+                    help='skip the rocprofv3 counter passes for roofline.traffic')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
@@ -206,9 +295,25 @@ def main():
         sys.exit(2)
 
     t0 = time.perf_counter()
-    plan, states = build_ensemble(args.replicas, min(16, os.cpu_count() or 1), args.ic_cache)
+    cache = args.ic_cache
+    own_cache = None
+    if args.traffic and not cache:
+        import tempfile
+        fd, own_cache = tempfile.mkstemp(prefix='rds_ic_', suffix='.npz')
+        os.close(fd)
+        os.unlink(own_cache)
+        cache = own_cache
+    plan, states = build_ensemble(args.replicas, min(16, os.cpu_count() or 1), cache)
     print('# built %d replica states in %.1f s' % (len(plan), time.perf_counter() - t0),
           file=sys.stderr, flush=True)
+    traffic = None
+    traffic_tiling = (args.tile_rows if args.tile_rows > 1 else 2,
+                      args.tile_particles if args.tile_particles > 0 else 56)
+    if args.traffic:
+        traffic = measure_traffic(cache, traffic_tiling[0], traffic_tiling[1], args.replicas)
+        print('# traffic: %s' % traffic, file=sys.stderr, flush=True)
+    if own_cache and os.path.exists(own_cache):
+        os.unlink(own_cache)
 
     import torch
     from pyrdsim_amd.engine import Engine
@@ -256,21 +361,33 @@ def main():
     else:
         force_bytes = FORCE_BYTES_PER_PARTICLE * n_tot + FORCE_BYTES_PER_CELL * n_cells
     achieved = force_bytes / (force_ms * 1e-3) / 1e9
+    traffic_bytes = int(traffic['FETCH_SIZE'] + traffic['WRITE_SIZE']) if traffic else None
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_bytes,
+                    traffic_read=int(traffic['FETCH_SIZE']) if traffic else None,
+                    traffic_write=int(traffic['WRITE_SIZE']) if traffic else None,
+                    traffic_tiling=list(traffic_tiling) if traffic else None,
                     kernel='k_force (pair forces + Euler-Maruyama + binning)',
                     algorithmic_bytes=int(force_bytes),
                     avg_ms={k: round(v[0], 5) for k, v in kt.items() if v[1]})
-    x_mixed, t_mixed = eng.state()
-    steps_done = eng.step
     tiling = eng.tiling()
     del eng
     torch.cuda.synchronize()
 
     accuracy = None
+    compensated = None
     if not args.no_accuracy:
-        accuracy = accuracy_leg(plan, states, models, reps, x_mixed, t_mixed, steps_done,
-                                args.accuracy_replicas)
+        # throughput of the same batched run with compensated FP32 positions
+        ec = Engine(models, reps, precision='compensated', graph_steps=args.graph_steps,
+                    tile_rows=tiling[0], tile_particles=tiling[1])
+        ec.set_state(x0, t0s)
+        secs_c = time_engine(ec, args.steps, args.warmup)
+        compensated = dict(value=n_tot * args.steps / secs_c, unit='particle-steps/s',
+                           ms_per_step=1e3 * secs_c / args.steps)
+        del ec
+        torch.cuda.synchronize()
+        accuracy = accuracy_leg(plan, states, models, reps, x0, t0s, args.graph_steps, tiling,
+                                args.accuracy_steps, args.accuracy_replicas)
         print('# accuracy: %s' % accuracy, file=sys.stderr, flush=True)
 
     plain = None
@@ -333,6 +450,7 @@ def main():
         'plain': plain,
         'batched_vs_plain': (value / plain['value']) if plain else None,
         'accuracy': accuracy,
+        'compensated_positions': compensated,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))

@@ -44,8 +44,12 @@ enum {
   RDS_E_STREAM = -5    /* injected random stream exhausted */
 };
 
-/* arithmetic of the pair forces; positions are FP64 in both modes */
-enum { RDS_FP64 = 0, RDS_MIXED = 1 };
+/* Precision.  RDS_FP64: FP64 pair forces and positions.  RDS_MIXED: FP32
+ * pair forces, FP64 positions and update.  RDS_COMPENSATED: FP32 pair forces
+ * and compensated FP32 positions -- each coordinate an unevaluated sum of two
+ * floats, differenced and updated with float arithmetic only (two-sum), read
+ * back as the exact FP64 value of the pair; cell search only (skin = 0). */
+enum { RDS_FP64 = 0, RDS_MIXED = 1, RDS_COMPENSATED = 2 };
 
 /* source of the Gaussian noise and of the reaction uniforms */
 enum { RDS_RNG_PHILOX = 0, RDS_RNG_INJECTED = 1 };
@@ -112,7 +116,7 @@ typedef struct {
   int32_t n_models, n_replicas;
   const rds_model* models;      /* host */
   const rds_replica* replicas;  /* host */
-  int32_t precision;            /* RDS_FP64 | RDS_MIXED */
+  int32_t precision;            /* RDS_FP64 | RDS_MIXED | RDS_COMPENSATED */
   int32_t rng;                  /* RDS_RNG_PHILOX | RDS_RNG_INJECTED */
   int32_t reaction_prob;        /* RDS_PROB_LINEAR | RDS_PROB_EXP */
   int32_t has_molecules;        /* set_state passes molecule numbers */

@@ -17,7 +17,7 @@ MAX_REACTIONS = 8
 ABI_VERSION = 2
 
 OK, E_ARG, E_HIP, E_BOUNDS, E_STATE, E_STREAM = 0, -1, -2, -3, -4, -5
-FP64, MIXED = 0, 1
+FP64, MIXED, COMPENSATED = 0, 1, 2
 RNG_PHILOX, RNG_INJECTED = 0, 1
 PROB_LINEAR, PROB_EXP = 0, 1
 LJ, TANH, YUKAWA, HARMONIC, PIECEWISE_LINEAR = 1, 2, 3, 4, 5

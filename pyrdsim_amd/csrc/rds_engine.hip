@@ -110,6 +110,8 @@ struct DRep {
   double fpre[MAXT], npre[MAXT], rate[MAXRX];
   double dt;
   double rl2;             // list mode: (largest pair cutoff + skin)^2
+  float Lh[2], Ll[2];     // box lengths as float pairs (compensated positions)
+  float fph[MAXT], fpl[MAXT], nph[MAXT], npl[MAXT];  // prefactors as float pairs
 };
 
 struct DStatus {
@@ -174,6 +176,7 @@ struct StepArgs {
   double2* xref;        // position of each slot at the last rebuild
   const int* scell;     // cell of each slot at the last rebuild (null: cell mode)
   double hskin2;        // (skin / 2)^2
+  int cpos;             // positions stored as float pairs (compensated FP32)
 };
 
 // ---------------------------------------------------------------------------
@@ -232,6 +235,61 @@ __device__ __forceinline__ void philox_uniforms(const DRep& R, int pid, long lon
 // sign of a zero, which no distance sees), so the division is skipped there.
 __device__ __forceinline__ double min_image(double dx, double L) {
   return fabs(dx) < 0.25 * L ? dx : dx - L * round(dx / L);
+}
+
+// Compensated FP32 positions (RDS_COMPENSATED): each coordinate is the
+// unevaluated sum h + l of two floats, |l| <= ulp(h)/2, stored as
+// float4 {xh, yh, xl, yl} in the 16-byte slot of a double2.  Its value is
+// exactly (double)h + (double)l.  The step updates it with float arithmetic
+// only (Knuth's two-sum / Dekker's fast-two-sum); everything else reads the
+// exact FP64 value.
+struct DF {
+  float h, l;
+};
+__device__ __forceinline__ DF two_sum(float a, float b) {
+#pragma clang fp contract(off)
+  const float s = a + b;
+  const float bb = s - a;
+  return DF{s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DF fast_two_sum(float a, float b) {  // |a| >= |b|
+#pragma clang fp contract(off)
+  const float s = a + b;
+  return DF{s, b - (s - a)};
+}
+__device__ __forceinline__ DF df_add(DF x, float y) {
+#pragma clang fp contract(off)
+  const DF t = two_sum(x.h, y);
+  return fast_two_sum(t.h, t.l + x.l);
+}
+__device__ __forceinline__ DF df_add(DF x, DF y) {
+#pragma clang fp contract(off)
+  const DF t = two_sum(x.h, y.h);
+  return fast_two_sum(t.h, t.l + (x.l + y.l));
+}
+// (a.h + a.l)(b.h + b.l) to about 2^-44 relative: the exact product of the
+// heads (fma two-product) plus the cross terms
+__device__ __forceinline__ DF df_mul(DF a, DF b) {
+#pragma clang fp contract(off)
+  const float p = a.h * b.h;
+  const float e = __builtin_fmaf(a.h, b.h, -p);
+  return fast_two_sum(p, e + (a.h * b.l + a.l * b.h));
+}
+__device__ __forceinline__ DF df_of(double x) {
+  const float h = (float)x;
+  return DF{h, (float)(x - (double)h)};
+}
+__device__ __forceinline__ bool df_lt(DF a, DF b) { return a.h < b.h || (a.h == b.h && a.l < b.l); }
+__device__ __forceinline__ double df_val(DF a) { return (double)a.h + (double)a.l; }
+__device__ __forceinline__ float4 df_split(double x, double y) {
+  const float xh = (float)x, yh = (float)y;
+  return make_float4(xh, yh, (float)(x - (double)xh), (float)(y - (double)yh));
+}
+// position of slot s as FP64, either storage
+__device__ __forceinline__ double2 rd_pos(const double2* __restrict__ p, int s, int cpos) {
+  if (!cpos) return p[s];
+  const float4 v = reinterpret_cast<const float4*>(p)[s];
+  return make_double2((double)v.x + (double)v.z, (double)v.y + (double)v.w);
 }
 
 __device__ __forceinline__ int cell_coord(double x, double cl, int l) {
@@ -387,12 +445,13 @@ __device__ __forceinline__ double2 near_cell(double2 q, const DRep& Rp, int cell
 }
 
 // Stages staged indices [p0, p0 + cap) into LDS.  scell (list mode): cells of
-// the last rebuild, whose nearest images are staged.
-template <int R, bool MOL>
+// the last rebuild, whose nearest images are staged.  cpos: the stored
+// positions are float pairs; SPM: stage float pairs (1) or FP64 (0).
+template <int R, bool MOL, int SPM = 0>
 __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
                            const double2* __restrict__ pos, const int* __restrict__ type,
                            const int* __restrict__ mol, double2* s_pos, int* s_ty, int* s_mol,
-                           const DRep& Rp, const int* __restrict__ scell) {
+                           const DRep& Rp, const int* __restrict__ scell, int cpos = 0) {
   const int t = threadIdx.x, nthr = blockDim.x;
   const int p1 = p0 + cap;
   for (int r = 0; r < NR; ++r) {
@@ -405,11 +464,12 @@ __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
       const double sx = sh.seg_sx[r][g];
       const int sl = sh.seg_slot[r][g] - so;
       for (int p = a + t; p < b; p += nthr) {
-        double2 q = pos[sl + p];
+        double2 q = rd_pos(pos, sl + p, cpos);
         if (scell) q = near_cell(q, Rp, scell[sl + p]);
         q.x = q.x + sx;
         q.y = q.y + sy;
-        s_pos[p - p0] = q;
+        if (SPM) reinterpret_cast<float4*>(s_pos)[p - p0] = df_split(q.x, q.y);
+        else s_pos[p - p0] = q;
         s_ty[p - p0] = type[sl + p];
         if (MOL) s_mol[p - p0] = mol[sl + p];
       }
@@ -435,7 +495,7 @@ __device__ __forceinline__ void home_windows(const TileSh<R>& sh, int wv, int b,
                                              int nrows, const DRep& Rp, int cx0, int W,
                                              const double2* __restrict__ pos,
                                              const int* __restrict__ scell, Home<R>& h,
-                                             double2& xi) {
+                                             double2& xi, int cpos = 0) {
   const int g = (b * R + wv) * 64 + lane;
   int q = 0;  // home row of this lane
 #pragma unroll
@@ -447,7 +507,7 @@ __device__ __forceinline__ void home_windows(const TileSh<R>& sh, int wv, int b,
   h.w = 0;
   h.own = -1;
   if (h.valid) {
-    xi = pos[h.s];
+    xi = rd_pos(pos, h.s, cpos);
     int cx;
     if (scell) {
       const int c = scell[h.s] - Rp.cbase;
@@ -587,7 +647,9 @@ __device__ __forceinline__ Real lj_fast(Real r2, Real pa, Real pb, int pw) {
   return pa * inv * s * (s - Real(1));
 }
 
-template <typename Real, bool MOL, int KM, int R, bool LIST>
+// PM: 0 = FP64 positions, 1 = compensated FP32 positions (float pairs; the
+// pair differences and the update are float arithmetic).
+template <typename Real, bool MOL, int KM, int R, bool LIST, int PM>
 __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, int pw) {
   constexpr bool FAST = KM != 0;
   extern __shared__ __align__(16) char smem[];
@@ -624,7 +686,8 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
   const bool resident = total <= CAPT;
   const int* scell = LIST ? A.scell : nullptr;
   if (resident) {
-    tile_stage<R, MOL>(sh, NR, 0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, scell);
+    tile_stage<R, MOL, PM>(sh, NR, 0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, scell,
+                           PM);
     __syncthreads();
   }
   long long step_next = 0;
@@ -633,7 +696,7 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
   for (int b = 0; b < sh.nbatch; ++b) {
     Home<R> h;
     double2 xi;
-    home_windows<R>(sh, wv, b, lane, nrows, Rp, cx0, W, A.pos, scell, h, xi);
+    home_windows<R>(sh, wv, b, lane, nrows, Rp, cx0, W, A.pos, scell, h, xi, PM);
     int ti = 0, mi = 0, p = 0;
     // position in the frame of the staged tile (list mode: nearest image to
     // the sorted cell); xi itself stays the integrator's coordinate
@@ -675,8 +738,8 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
     // never the particle itself nor of its molecule; stencil candidates are
     // filtered here: self / same molecule never interact, and a particle's only
     // zero-distance partner in an atomic model is itself.
-    auto pairv = [&](const double2 pj, int tj, int mj, bool live, bool listed) {
-      const Real dx = (Real)(pj.x - xs.x), dy = (Real)(pj.y - xs.y);
+    const float4 xsf = PM ? df_split(xs.x, xs.y) : make_float4(0.f, 0.f, 0.f, 0.f);
+    auto pairv = [&](const Real dx, const Real dy, int tj, int mj, bool live, bool listed) {
       const Real r2 = dx * dx + dy * dy;
       const bool other = (MOL && listed) ? true : (MOL ? (mj != mi) : (r2 > Real(0)));
       if (FAST) {
@@ -699,14 +762,26 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
       }
     };
     auto pair = [&](int j, bool live, bool listed) {
-      pairv(s_pos[j], s_ty[j], MOL ? s_mol[j] : 0, live, listed);
+      Real dx, dy;
+      if (PM) {
+        // difference of float pairs: (h_j - h_i) + (l_j - l_i)
+        const float4 q = reinterpret_cast<const float4*>(s_pos)[j];
+        dx = (Real)((q.x - xsf.x) + (q.z - xsf.z));
+        dy = (Real)((q.y - xsf.y) + (q.w - xsf.w));
+      } else {
+        const double2 pj = s_pos[j];
+        dx = (Real)(pj.x - xs.x);
+        dy = (Real)(pj.y - xs.y);
+      }
+      pairv(dx, dy, s_ty[j], MOL ? s_mol[j] : 0, live, listed);
     };
     const uint2* lp = A.nl4 + (LIST && h.valid ? h.s : 0);
     int kcur = 0;  // list cursor across LDS rounds
     for (int p0 = 0; p0 < total; p0 += CAPT) {
       if (!resident) {
         __syncthreads();
-        tile_stage<R, MOL>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, scell);
+        tile_stage<R, MOL, PM>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp,
+                               scell, PM);
         __syncthreads();
       }
       if (LIST) {
@@ -787,47 +862,88 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
       n0 = a0;
       n1 = a1;
     }
-    const double fp = Rp.fpre[ti];
-    const double np = A.include_rf * Rp.npre[ti];
-    double x = xi.x, y = xi.y;
-    {
-#pragma clang fp contract(off)
-      x = x + fp * Fx;
-      x = x + np * n0;
-      y = y + fp * Fy;
-      y = y + np * n1;
-    }
-    // boundary conditions, pyrdsim.pyx:2269-2292
-    double v[2] = {x, y};
+    double v[2];
+    if (PM) {
+      // compensated FP32: x + fp*F, then + np*xi, each increment a float-pair
+      // product and each addition a float-pair addition
+      const float4 hv = reinterpret_cast<const float4*>(A.pos)[s];
+      DF u2[2] = {DF{hv.x, hv.z}, DF{hv.y, hv.w}};
+      const DF fpd{Rp.fph[ti], Rp.fpl[ti]};
+      const DF npd = A.include_rf != 0.0 ? DF{Rp.nph[ti], Rp.npl[ti]} : DF{0.f, 0.f};
+      u2[0] = df_add(u2[0], df_mul(fpd, df_of(Fx)));
+      u2[0] = df_add(u2[0], df_mul(npd, df_of(n0)));
+      u2[1] = df_add(u2[1], df_mul(fpd, df_of(Fy)));
+      u2[1] = df_add(u2[1], df_mul(npd, df_of(n1)));
 #pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const double L = Rp.L[k];
-      double u = v[k];
-      if (!(k == 0 ? Rp.px : Rp.py)) {
-        if (u > L) u -= 2.0 * (u - L);
-        if (u < 0.0) u -= 2.0 * u;
-      } else {
-        // Python-style modulo as Cython emits it for doubles: fmod, then + L
-        // for a negative remainder.  A step moves far less than L, so fmod's
-        // exact remainder is u itself on (-L, L) and u - L on [L, 2L) (exact:
-        // L <= u < 2L); the general routine only for anything else.
-        double m;
-        if (u > -L && u < L) m = u;
-        else if (u >= L && u < 2.0 * L) m = u - L;
-        else m = fmod(u, L);
-        if (m != 0.0 && ((m < 0.0) != (L < 0.0))) m += L;
-        u = m;
+      for (int k = 0; k < 2; ++k) {
+        DF u = u2[k];
+        const DF Lk{Rp.Lh[k], Rp.Ll[k]};
+        const DF zero{0.f, 0.f};
+        if (!(k == 0 ? Rp.px : Rp.py)) {
+          // single reflection, as pyrdsim.pyx:2269-2281
+          if (df_lt(Lk, u)) u = df_add(DF{-u.h, -u.l}, DF{2.f * Lk.h, 2.f * Lk.l});
+          if (df_lt(u, zero)) u = DF{-u.h, -u.l};
+        } else {
+          if (df_lt(u, zero)) u = df_add(u, Lk);
+          else if (!df_lt(u, Lk)) u = df_add(u, DF{-Lk.h, -Lk.l});
+        }
+        u2[k] = u;
+        const double uv = df_val(u);
+        if (uv < 0.0 || uv > Rp.L[k] || uv != uv) {
+          unsigned long long key = ((unsigned long long)step_next << 32) |
+                                   (unsigned long long)(unsigned)(p * 2 + k);
+          unsigned long long old = atomicMin(&A.status->err_key, key);
+          if (key < old) A.status->err_val = uv;
+          A.status->bounds = 1;
+        }
+        v[k] = uv;
       }
-      if (u < 0.0 || u > L || u != u) {
-        unsigned long long key = ((unsigned long long)step_next << 32) |
-                                 (unsigned long long)(unsigned)(p * 2 + k);
-        unsigned long long old = atomicMin(&A.status->err_key, key);
-        if (key < old) A.status->err_val = u;
-        A.status->bounds = 1;
+      reinterpret_cast<float4*>(A.upos)[s] = make_float4(u2[0].h, u2[1].h, u2[0].l, u2[1].l);
+    } else {
+      const double fp = Rp.fpre[ti];
+      const double np = A.include_rf * Rp.npre[ti];
+      double x = xi.x, y = xi.y;
+      {
+#pragma clang fp contract(off)
+        x = x + fp * Fx;
+        x = x + np * n0;
+        y = y + fp * Fy;
+        y = y + np * n1;
       }
-      v[k] = u;
+      // boundary conditions, pyrdsim.pyx:2269-2292
+      v[0] = x;
+      v[1] = y;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const double L = Rp.L[k];
+        double u = v[k];
+        if (!(k == 0 ? Rp.px : Rp.py)) {
+          if (u > L) u -= 2.0 * (u - L);
+          if (u < 0.0) u -= 2.0 * u;
+        } else {
+          // Python-style modulo as Cython emits it for doubles: fmod, then + L
+          // for a negative remainder.  A step moves far less than L, so fmod's
+          // exact remainder is u itself on (-L, L) and u - L on [L, 2L) (exact:
+          // L <= u < 2L); the general routine only for anything else.
+          double m;
+          if (u > -L && u < L) m = u;
+          else if (u >= L && u < 2.0 * L) m = u - L;
+          else m = fmod(u, L);
+          if (m != 0.0 && ((m < 0.0) != (L < 0.0))) m += L;
+          u = m;
+        }
+        if (u < 0.0 || u > L || u != u) {
+          unsigned long long key = ((unsigned long long)step_next << 32) |
+                                   (unsigned long long)(unsigned)(p * 2 + k);
+          unsigned long long old = atomicMin(&A.status->err_key, key);
+          if (key < old) A.status->err_val = u;
+          A.status->bounds = 1;
+        }
+        v[k] = u;
+      }
+
+      A.upos[s] = make_double2(v[0], v[1]);
     }
-    A.upos[s] = make_double2(v[0], v[1]);
     if (LIST) {
       // same slot; request a rebuild once the particle has moved more than
       // skin/2 (minimum image on periodic axes) from its position at the last one
@@ -973,12 +1089,13 @@ __global__ __launch_bounds__(64) void k_count(const StepArgs A) {
     for (int b = 0; b < sh.nbatch; ++b) {
       Home<1> h;
       double2 xi;
-      home_windows<1>(sh, 0, b, lane, 1, Rp, cx0, W, A.pos, nullptr, h, xi);
+      home_windows<1>(sh, 0, b, lane, 1, Rp, cx0, W, A.pos, nullptr, h, xi, A.cpos);
       int mi = (MOL && h.valid) ? A.mol[h.s] : 0;
       int cnt = 0;
       for (int p0 = 0; p0 < total; p0 += CAPT) {
         __syncthreads();
-        tile_stage<1, MOL>(sh, 3, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, nullptr);
+        tile_stage<1, MOL>(sh, 3, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, nullptr,
+                           A.cpos);
         __syncthreads();
         for (int d = 0; d < 3; ++d) {
           const int lo = max(h.lo[d], p0) - p0;
@@ -1049,7 +1166,7 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
 #pragma unroll
     for (int q = 0; q < MAXRX; ++q) cnt[q] = 0;
     if (need) {
-      const double2 xi = A.pos[s];
+      const double2 xi = rd_pos(A.pos, s, A.cpos);
       const int mi = MOL ? A.mol[s] : 0;
       // the cell the slot is sorted into (list mode: of the last rebuild; cells
       // are then at least range + skin wide, so the 3x3 stencil still holds
@@ -1080,7 +1197,7 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
           for (int j = A.cstart[c]; j < j1; ++j) {
             if (j == s) continue;
             if (MOL && A.mol[j] == mi) continue;
-            const double2 xj = A.pos[j];
+            const double2 xj = rd_pos(A.pos, j, A.cpos);
             double dx = xj.x - xi.x, dy = xj.y - xi.y;
             if (Rp.px) dx = min_image(dx, Rp.L[0]);
             if (Rp.py) dy = min_image(dy, Rp.L[1]);
@@ -1152,15 +1269,15 @@ __global__ void k_bonds(int n_total, const DRep* __restrict__ rep, const int* __
                         const double2* __restrict__ pos, const int* __restrict__ pid,
                         const int* __restrict__ slot_of, const int* __restrict__ boff,
                         const int* __restrict__ bpart, const double* __restrict__ br0,
-                        const double* __restrict__ bu0, double2* __restrict__ fbond) {
+                        const double* __restrict__ bu0, double2* __restrict__ fbond, int cpos) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_total) return;
   int p = pid[s];
   const DRep& R = rep[rep_of[p]];
-  double2 xi = pos[s];
+  double2 xi = rd_pos(pos, s, cpos);
   double fx = 0.0, fy = 0.0;
   for (int b = boff[p]; b < boff[p + 1]; ++b) {
-    double2 xj = pos[slot_of[bpart[b]]];
+    double2 xj = rd_pos(pos, slot_of[bpart[b]], cpos);
     double dx = xj.x - xi.x, dy = xj.y - xi.y;
     if (R.px) dx = min_image(dx, R.L[0]);
     if (R.py) dy = min_image(dy, R.L[1]);
@@ -1182,11 +1299,11 @@ __global__ void k_bonds(int n_total, const DRep* __restrict__ rep, const int* __
 __global__ void k_bin(int n_total, const DRep* __restrict__ rep, const int* __restrict__ rep_of,
                       const double2* __restrict__ pos,
                       int* __restrict__ ucell, int* __restrict__ urank, int* __restrict__ count,
-                      const DStatus* cond) {
+                      const DStatus* cond, int cpos) {
   if (cond && !cond->rebuild) return;
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_total; s += gridDim.x * blockDim.x) {
     const DRep& R = rep[rep_of[s]];
-    double2 x = pos[s];
+    double2 x = rd_pos(pos, s, cpos);
     int c = R.cbase + cell_coord(x.y, R.cl[1], R.ly) * R.lx + cell_coord(x.x, R.cl[0], R.lx);
     ucell[s] = c;
     urank[s] = atomicAdd(&count[c], 1);
@@ -1336,22 +1453,22 @@ __global__ void k_request(DStatus* status) { status->rebuild = 1; }
 
 __global__ void k_gather_out(int n_total, const double2* __restrict__ pos,
                              const int* __restrict__ type, const int* __restrict__ pid,
-                             double2* __restrict__ xo, int* __restrict__ to) {
+                             double2* __restrict__ xo, int* __restrict__ to, int cpos) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_total) return;
   int p = pid[s];
-  if (xo) xo[p] = pos[s];
+  if (xo) xo[p] = rd_pos(pos, s, cpos);
   if (to) to[p] = type[s];
 }
 
 __global__ void k_cells_out(int n_total, const DRep* __restrict__ rep,
                             const int* __restrict__ rep_of, const double2* __restrict__ pos,
-                            const int* __restrict__ pid, int* __restrict__ out) {
+                            const int* __restrict__ pid, int* __restrict__ out, int cpos) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_total) return;
   int p = pid[s];
   const DRep& R = rep[rep_of[p]];
-  double2 x = pos[s];
+  double2 x = rd_pos(pos, s, cpos);
   out[p] = R.cbase + cell_coord(x.y, R.cl[1], R.ly) * R.lx + cell_coord(x.x, R.cl[0], R.lx);
 }
 
@@ -1366,10 +1483,12 @@ __global__ __launch_bounds__(256) void k_fill_lds(unsigned pattern, int words) {
 
 __global__ void k_load(int n_total, const double2* __restrict__ x, const int* __restrict__ t,
                        const int* __restrict__ m, double2* __restrict__ pos,
-                       int* __restrict__ type, int* __restrict__ pid, int* __restrict__ mol) {
+                       int* __restrict__ type, int* __restrict__ pid, int* __restrict__ mol,
+                       int cpos) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_total) return;
-  pos[s] = x[s];
+  if (cpos) reinterpret_cast<float4*>(pos)[s] = df_split(x[s].x, x[s].y);
+  else pos[s] = x[s];
   type[s] = t[s];
   pid[s] = s;
   if (mol) mol[s] = m[s];
@@ -1394,6 +1513,7 @@ struct rds_ctx {
   hipStream_t stream = nullptr;
   bool own_stream = true;
   int precision = RDS_MIXED, rng = RDS_RNG_PHILOX, prob_mode = RDS_PROB_LINEAR;
+  int cpos = 0;  // compensated FP32 positions
   int has_mol = 0;
   int graph_steps = 0;
   double include_rf = 1.0;
@@ -1499,41 +1619,43 @@ StepArgs make_args(rds_ctx* c, const Buf& cur) {
   a.xref = c->xref;
   a.scell = c->list ? c->scell : nullptr;
   a.hskin2 = 0.25 * c->skin * c->skin;
+  a.cpos = c->cpos;
   return a;
 }
 
-template <typename Real, bool MOL, int KM, bool LIST>
+template <typename Real, bool MOL, int KM, bool LIST, int PM>
 void launch_force_r(rds_ctx* c, const StepArgs& a, int mode) {
   dim3 g((unsigned)c->n_tiles);
   const size_t sm = c->smem_bytes;
   switch (c->rows) {
-    case 4: hipLaunchKernelGGL((k_force<Real, MOL, KM, 4, LIST>), g, dim3(256), sm, c->stream, a, mode, c->pw); break;
-    case 2: hipLaunchKernelGGL((k_force<Real, MOL, KM, 2, LIST>), g, dim3(128), sm, c->stream, a, mode, c->pw); break;
-    default: hipLaunchKernelGGL((k_force<Real, MOL, KM, 1, LIST>), g, dim3(64), sm, c->stream, a, mode, c->pw); break;
+    case 4: hipLaunchKernelGGL((k_force<Real, MOL, KM, 4, LIST, PM>), g, dim3(256), sm, c->stream, a, mode, c->pw); break;
+    case 2: hipLaunchKernelGGL((k_force<Real, MOL, KM, 2, LIST, PM>), g, dim3(128), sm, c->stream, a, mode, c->pw); break;
+    default: hipLaunchKernelGGL((k_force<Real, MOL, KM, 1, LIST, PM>), g, dim3(64), sm, c->stream, a, mode, c->pw); break;
   }
 }
 
-template <typename Real, bool MOL, bool LIST>
+template <typename Real, bool MOL, bool LIST, int PM>
 void launch_force_m(rds_ctx* c, const StepArgs& a, int mode) {
-  if (!c->fast) launch_force_r<Real, MOL, 0, LIST>(c, a, mode);
-  else if (c->pw == 3) launch_force_r<Real, MOL, 2, LIST>(c, a, mode);
-  else launch_force_r<Real, MOL, 1, LIST>(c, a, mode);
+  if (!c->fast) launch_force_r<Real, MOL, 0, LIST, PM>(c, a, mode);
+  else if (c->pw == 3) launch_force_r<Real, MOL, 2, LIST, PM>(c, a, mode);
+  else launch_force_r<Real, MOL, 1, LIST, PM>(c, a, mode);
 }
 
-template <typename Real, bool LIST>
+template <typename Real, bool LIST, int PM>
 void launch_force_l(rds_ctx* c, const StepArgs& a, int mode) {
-  if (c->has_mol) launch_force_m<Real, true, LIST>(c, a, mode);
-  else launch_force_m<Real, false, LIST>(c, a, mode);
+  if (c->has_mol) launch_force_m<Real, true, LIST, PM>(c, a, mode);
+  else launch_force_m<Real, false, LIST, PM>(c, a, mode);
 }
 
 template <typename Real>
 void launch_force_p(rds_ctx* c, const StepArgs& a, int mode) {
-  if (c->list) launch_force_l<Real, true>(c, a, mode);
-  else launch_force_l<Real, false>(c, a, mode);
+  if (c->list) launch_force_l<Real, true, 0>(c, a, mode);
+  else launch_force_l<Real, false, 0>(c, a, mode);
 }
 
 void launch_force(rds_ctx* c, const StepArgs& a, int mode) {
   if (c->precision == RDS_FP64) launch_force_p<double>(c, a, mode);
+  else if (c->cpos) launch_force_l<float, false, 1>(c, a, mode);  // cell mode only
   else launch_force_p<float>(c, a, mode);
 }
 
@@ -1566,7 +1688,7 @@ void enqueue_rebuild(rds_ctx* c, double2* pos, int* type, int advance, Mark&& ma
   int* mol = c->has_mol ? c->buf[0].mol : nullptr;
   mark(KT_LIST, true);
   hipLaunchKernelGGL(k_bin, dim3(nblk_gs(N)), dim3(256), 0, c->stream, N, c->d_rep, c->d_rep_of,
-                     pos, c->ucell, c->urank, c->count, c->d_status);
+                     pos, c->ucell, c->urank, c->count, c->d_status, c->cpos);
   mark(KT_LIST, false);
   mark(KT_SCAN, true);
   hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
@@ -1626,7 +1748,7 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
     mark(KT_BONDS, true);
     hipLaunchKernelGGL(k_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
                        (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, c->slot_of,
-                       c->boff, c->bpart, c->br0, c->bu0, c->fbond);
+                       c->boff, c->bpart, c->br0, c->bu0, c->fbond, c->cpos);
     mark(KT_BONDS, false);
   }
   StepArgs a = make_args(c, cur);
@@ -1925,8 +2047,11 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   *out = nullptr;
   if (cfg->n_models < 1 || cfg->n_replicas < 1 || !cfg->models || !cfg->replicas)
     return fail(RDS_E_ARG, "need at least one model and one replica");
-  if (cfg->precision != RDS_FP64 && cfg->precision != RDS_MIXED)
+  if (cfg->precision != RDS_FP64 && cfg->precision != RDS_MIXED &&
+      cfg->precision != RDS_COMPENSATED)
     return fail(RDS_E_ARG, "unknown precision %d", cfg->precision);
+  if (cfg->precision == RDS_COMPENSATED && cfg->skin > 0.0)
+    return fail(RDS_E_ARG, "compensated FP32 positions run with skin = 0 (cell search every step)");
   if (cfg->rng != RDS_RNG_PHILOX && cfg->rng != RDS_RNG_INJECTED)
     return fail(RDS_E_ARG, "unknown rng %d", cfg->rng);
   rds_ctx* c = new rds_ctx();
@@ -1936,6 +2061,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     return code;
   };
   c->precision = cfg->precision;
+  c->cpos = cfg->precision == RDS_COMPENSATED ? 1 : 0;
   c->rng = cfg->rng;
   c->prob_mode = cfg->reaction_prob;
   c->has_mol = cfg->has_molecules ? 1 : 0;
@@ -2031,6 +2157,16 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     }
     for (int q = 0; q < MAXRX; ++q) d.rate[q] = s.rate[q];
     d.dt = s.dt;
+    for (int k = 0; k < 2; ++k) {
+      d.Lh[k] = (float)s.L[k];
+      d.Ll[k] = (float)(s.L[k] - (double)d.Lh[k]);
+    }
+    for (int t = 0; t < MAXT; ++t) {
+      d.fph[t] = (float)s.force_prefac[t];
+      d.fpl[t] = (float)(s.force_prefac[t] - (double)d.fph[t]);
+      d.nph[t] = (float)s.noise_prefac[t];
+      d.npl[t] = (float)(s.noise_prefac[t] - (double)d.nph[t]);
+    }
     {
       const double rl = std::sqrt(c->h_models[s.model].rc2max) + c->skin;
       d.rl2 = rl * rl;
@@ -2171,7 +2307,7 @@ int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32
   Buf& b = c->buf[1];
   hipLaunchKernelGGL(k_load, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, (const double2*)x,
                      types, c->has_mol ? mol : nullptr, a.pos, a.type, a.pid,
-                     c->has_mol ? a.mol : nullptr);
+                     c->has_mol ? a.mol : nullptr, c->cpos);
   DStatus st;
   memset(&st, 0, sizeof st);
   st.err_key = ~0ull;
@@ -2181,7 +2317,7 @@ int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32
     force_rebuild(c);
   } else {
     hipLaunchKernelGGL(k_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep,
-                       c->d_rep_of, a.pos, c->ucell, c->urank, c->count, nullptr);
+                       c->d_rep_of, a.pos, c->ucell, c->urank, c->count, nullptr, c->cpos);
     hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep,
                        c->count, c->cstart, c->d_step, 0, c->d_status, 0);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos,
@@ -2205,7 +2341,7 @@ int rds_get_state(rds_ctx* c, double* x, int32_t* types) {
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   Buf& cur = c->buf[c->parity];
   hipLaunchKernelGGL(k_gather_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
-                     (int)c->n_total, cur.pos, cur.type, cur.pid, (double2*)x, types);
+                     (int)c->n_total, cur.pos, cur.type, cur.pid, (double2*)x, types, c->cpos);
   HIP_TRY(hipGetLastError());
   return RDS_OK;
 }
@@ -2290,7 +2426,7 @@ int rds_compute_forces(rds_ctx* c, double* forces) {
   if (c->any_bonds)
     hipLaunchKernelGGL(k_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
                        (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, c->slot_of,
-                       c->boff, c->bpart, c->br0, c->bu0, c->fbond);
+                       c->boff, c->bpart, c->br0, c->bu0, c->fbond, c->cpos);
   StepArgs a = make_args(c, cur);
   a.fout = (double2*)forces;
   launch_force(c, a, MODE_FORCE);
@@ -2303,7 +2439,7 @@ int rds_cell_ids(rds_ctx* c, int32_t* cells) {
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   Buf& cur = c->buf[c->parity];
   hipLaunchKernelGGL(k_cells_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
-                     (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, cells);
+                     (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, cells, c->cpos);
   HIP_TRY(hipGetLastError());
   return RDS_OK;
 }

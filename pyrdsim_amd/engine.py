@@ -12,7 +12,7 @@ import torch
 
 from . import _lib as L
 
-PRECISIONS = {'fp64': L.FP64, 'mixed': L.MIXED}
+PRECISIONS = {'fp64': L.FP64, 'mixed': L.MIXED, 'compensated': L.COMPENSATED}
 RNGS = {'philox': L.RNG_PHILOX, 'injected': L.RNG_INJECTED}
 PROBS = {'linear': L.PROB_LINEAR, 'exp': L.PROB_EXP}
 

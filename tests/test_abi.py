@@ -75,3 +75,34 @@ def test_errors_without_context():
     assert lib.rds_run(None, 1) == _lib.E_ARG
     assert b'null' in lib.rds_last_error()
     assert lib.rds_sync(None, None) == _lib.E_ARG
+
+
+def _config(precision, skin):
+    model = _lib.Model()
+    model.n_types = 1
+    rep = _lib.Replica()
+    rep.n_particles = 16
+    rep.L[0] = rep.L[1] = 10.0
+    rep.periodic[0] = rep.periodic[1] = 1
+    rep.dt = 1e-4
+    cfg = _lib.Config()
+    cfg.n_models = cfg.n_replicas = 1
+    keep = (model, rep)
+    cfg.models = C.pointer(model)
+    cfg.replicas = C.pointer(rep)
+    cfg.precision = precision
+    cfg.skin = skin
+    return cfg, keep
+
+
+def test_config_errors_before_any_device_call():
+    """Rejected configurations return RDS_E_ARG before touching a GPU."""
+    lib = _lib.load()
+    ctx = C.c_void_p()
+    for prec, skin, text in ((7, 0.0, b'unknown precision'),
+                             (_lib.COMPENSATED, 0.5, b'compensated'),
+                             (_lib.MIXED, -1.0, b'skin')):
+        cfg, keep = _config(prec, skin)
+        assert lib.rds_ctx_create(C.byref(cfg), C.byref(ctx)) == _lib.E_ARG
+        assert text in lib.rds_last_error(), lib.rds_last_error()
+        assert not ctx.value

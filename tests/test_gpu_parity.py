@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize('name', ['lj1', 'kalj', 'wall', 'patchy'])
-@pytest.mark.parametrize('precision,rtol', [('fp64', 1e-10), ('mixed', 1e-5)])
+@pytest.mark.parametrize('precision,rtol', [('fp64', 1e-10), ('mixed', 1e-5), ('compensated', 1e-5)])
 def test_forces_match_oracle(gpu, name, precision, rtol):
     s = small(name)
     if name == 'wall':

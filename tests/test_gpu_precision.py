@@ -1,8 +1,13 @@
 """Precision modes and tuning on the GPU.
 
-* FP32 pair forces with FP64 positions reproduce the FP64 path's radial
-  distribution function within 1% (L1, relative) and its activated fraction,
-  averaged over the second half of a run, with the same Philox streams;
+* FP32 pair forces with FP64 positions ('mixed') and with compensated FP32
+  positions ('compensated') reproduce the FP64 path's radial distribution
+  function within 1% (L1, relative) and its activated fraction, averaged over
+  the second half of a run, with the same Philox streams;
+* against an FP64 trajectory with the same injected noise, compensated
+  positions stay as close as FP64 positions do (both limited by the FP32 pair
+  arithmetic), far below the per-step rounding that plain FP32 coordinates
+  would carry (half an ulp of the box length);
 * re-tiling a live context (what the autotuner does) changes nothing but the
   work decomposition.
 """
@@ -34,13 +39,44 @@ def test_mixed_precision_rdf_within_one_percent(gpu, name):
     s = small(name, n=6000, seed=21)
     steps, every = 4000, 200
     res = {}
-    for prec in ('fp64', 'mixed'):
+    for prec in ('fp64', 'mixed', 'compensated'):
         eng = engine_for(s, precision=prec, seed=77, graph_steps=20)
         res[prec] = _averaged(eng, s, steps, every)
         eng.close()
-    err = analysis.rdf_error(res['mixed'][0], res['fp64'][0])
-    assert err <= 0.01, err
-    assert abs(res['mixed'][1] - res['fp64'][1]) <= 0.01
+    for prec in ('mixed', 'compensated'):
+        err = analysis.rdf_error(res[prec][0], res['fp64'][0])
+        assert err <= 0.01, (prec, err)
+        assert abs(res[prec][1] - res['fp64'][1]) <= 0.01, prec
+
+
+@pytest.mark.parametrize('name', ['kalj', 'wall', 'patchy'])
+def test_compensated_positions_track_fp64(gpu, name):
+    from test_gpu_parity import _injected
+    s = small(name)
+    steps = 200
+    normals, u1, u2 = _injected(len(s['x']), steps, 31)
+    if name == 'wall':
+        u1 *= 1e-3
+    out = {}
+    for prec in ('fp64', 'mixed', 'compensated'):
+        eng = engine_for(s, precision=prec, rng='injected')
+        eng.set_injected(normals, u1, u2, first_step=1)
+        eng.run(steps)
+        eng.sync()
+        out[prec] = eng.state()
+        eng.close()
+    dev = {}
+    for prec in ('mixed', 'compensated'):
+        d = out[prec][0] - out['fp64'][0]
+        d = d - s['L'] * np.round(d / s['L']) * (s['periodic'] > 0)
+        dev[prec] = float(np.abs(d).max())
+        assert np.array_equal(out[prec][1], out['fp64'][1]), prec
+    # plain FP32 coordinates round every step: a random walk of rounding errors
+    # of about half an ulp of the box length
+    plain = float(np.spacing(np.float32(max(s['L'])))) / 2 * np.sqrt(steps / 3.0)
+    print(name, dev, 'plain FP32 coordinates, expected:', plain)
+    assert dev['compensated'] <= 3 * dev['mixed'] + 1e-12, dev
+    assert dev['compensated'] < 1e-2 * plain, (dev, plain)
 
 
 def test_retiling_keeps_state_and_forces(gpu):
