@@ -177,6 +177,8 @@ struct StepArgs {
   const int* scell;     // cell of each slot at the last rebuild (null: cell mode)
   double hskin2;        // (skin / 2)^2
   int cpos;             // positions stored as float pairs (compensated FP32)
+  const int* boff;      // bond CSR by pid (null without bonds)
+  const int* bpart;
 };
 
 // ---------------------------------------------------------------------------
@@ -1160,8 +1162,16 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
     const int p = A.pid[s];
     const long long step = *A.step;  // already advanced: the reference's l+1
     bool need = false;
-    for (int q = 0; q < nrx; ++q)
-      if (M.rx[q].kind != RDS_RX_LINEAR && M.rx[q].reactant == ti) need = true;
+    unsigned nbr_types = 0u;  // neighbour types that any of ti's reactions counts
+    for (int q = 0; q < nrx; ++q) {
+      const DRx& rx = M.rx[q];
+      if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
+      need = true;
+      const int a = rx.kind == RDS_RX_CATALYZED ? rx.catalyst : rx.inh;
+      const int b = rx.kind == RDS_RX_CATALYZED ? -1 : rx.inh2;
+      if (a >= 0 && a < 32) nbr_types |= 1u << a;
+      if (b >= 0 && b < 32) nbr_types |= 1u << b;
+    }
     int cnt[MAXRX];
 #pragma unroll
     for (int q = 0; q < MAXRX; ++q) cnt[q] = 0;
@@ -1196,13 +1206,24 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
           const int j1 = A.cstart[c + 1];
           for (int j = A.cstart[c]; j < j1; ++j) {
             if (j == s) continue;
-            if (MOL && A.mol[j] == mi) continue;
+            // only neighbours of a counted type need a distance
+            const int tj = A.type[j];
+            if (tj < 0 || tj >= 32 || !((nbr_types >> tj) & 1u)) continue;
+            // the reference's neighbour list holds other molecules' particles
+            // and the bonded partners (pyrdsim.pyx:1513, 1523)
+            if (MOL && A.mol[j] == mi) {
+              bool bonded = false;
+              if (A.boff) {
+                const int pj = A.pid[j];
+                for (int b = A.boff[p]; b < A.boff[p + 1]; ++b) bonded |= A.bpart[b] == pj;
+              }
+              if (!bonded) continue;
+            }
             const double2 xj = rd_pos(A.pos, j, A.cpos);
             double dx = xj.x - xi.x, dy = xj.y - xi.y;
             if (Rp.px) dx = min_image(dx, Rp.L[0]);
             if (Rp.py) dy = min_image(dy, Rp.L[1]);
-            const double d = sqrt(dx * dx + dy * dy);
-            const int tj = A.type[j];
+            const double d2 = dx * dx + dy * dy;
 #pragma unroll
             for (int q = 0; q < MAXRX; ++q) {
               if (q >= nrx) break;
@@ -1210,7 +1231,11 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
               if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
               const bool match = (rx.kind == RDS_RX_CATALYZED) ? (tj == rx.catalyst)
                                                                : (tj == rx.inh || tj == rx.inh2);
-              if (match && d < rx.r0) cnt[q] += 1;
+              // d < r0 with d = sqrt(d2) as the reference computes it; away
+              // from r0^2 (relative 1e-14) d2 alone decides it
+              const bool within = d2 < rx.r0sq_lo ||
+                                  (d2 <= rx.r0sq_hi && sqrt(d2) < rx.r0);
+              if (match && within) cnt[q] += 1;
             }
           }
         }
@@ -1620,6 +1645,8 @@ StepArgs make_args(rds_ctx* c, const Buf& cur) {
   a.scell = c->list ? c->scell : nullptr;
   a.hskin2 = 0.25 * c->skin * c->skin;
   a.cpos = c->cpos;
+  a.boff = c->any_bonds ? c->boff : nullptr;
+  a.bpart = c->any_bonds ? c->bpart : nullptr;
   return a;
 }
 

@@ -252,3 +252,33 @@ def test_no_reads_of_unstaged_lds(gpu, rows, per_tile, precision):
     eng.sync()
     x, _ = eng.state()
     assert np.isfinite(x).all()
+
+
+def test_catalysis_counts_bonded_partners(gpu):
+    """The reference counts catalysts over its neighbour list, which holds the
+    bonded partners of the same molecule (pyrdsim.pyx:1513) but no other
+    same-molecule pair.  With the catalysis radius above the patch-patch bond
+    length (0.87 sigma), bonded active patches count: events equal the
+    oracle's, step by step."""
+    s = small('patchy')
+    s['types'][1::4] = 2
+    s['types'][2::4] = 2
+    for rx in s['model']['reactions']:
+        if rx['type'] == 'locally_catalyzed':
+            rx['r0'] = 1.0
+            rx['rate'] = 500.0
+    steps = 20
+    normals, u1, u2 = _injected(len(s['x']), steps, 41)
+    eng = engine_for(s, precision='fp64', rng='injected')
+    eng.set_injected(normals, u1, u2, first_step=1)
+    o = oracle_for(s, rng='injected')
+    o.set_injected(normals, u1, u2, first_step=1)
+    o.init()
+    eng.run(steps)
+    st = eng.sync()
+    o.run(1, steps)
+    xg, tg = eng.state()
+    xo, to, _ = o.state()
+    assert np.array_equal(tg, to), int((tg != to).sum())
+    _, _, ev = o.reaction_info()
+    assert st.reaction_events == ev and ev > 0
