@@ -646,7 +646,19 @@ __device__ __forceinline__ Real lj_fast(Real r2, Real pa, Real pb, int pw) {
     s = bb;
     for (int k = 1; k < pw; ++k) s *= bb;
   }
-  return pa * inv * s * (s - Real(1));
+  // pa / r^2 * s (s - 1), with s (s - 1) = s*s - s in one fused operation
+  return (pa * inv) * fma(s, s, -s);
+}
+
+// 0 < r2 <= rc2 for r2 >= +0 or NaN and rc2 >= 0, as one unsigned compare: the
+// bit patterns of non-negative floats are ordered like their values, +0 maps
+// to 0 (and wraps to the largest pattern), NaN lies above every finite value.
+// rc2 = 0 (no term for the type pair) admits nothing.
+__device__ __forceinline__ bool in_range(float r2, float rc2) {
+  return __float_as_uint(r2) - 1u < __float_as_uint(rc2);
+}
+__device__ __forceinline__ bool in_range(double r2, double rc2) {
+  return r2 > 0.0 && r2 <= rc2;
 }
 
 // PM: 0 = FP64 positions, 1 = compensated FP32 positions (float pairs; the
@@ -721,8 +733,9 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
         nlist = 0;
       }
     }
-    // FAST: the (ti, tj) constants for tj = 0, 1 in registers
-    Real pa0 = 0, pb0 = 0, pc0 = -1, pa1 = 0, pb1 = 0, pc1 = -1;
+    // FAST: the (ti, tj) constants for tj = 0, 1 in registers (cutoff^2 = 0:
+    // no term for the pair)
+    Real pa0 = 0, pb0 = 0, pc0 = 0, pa1 = 0, pb1 = 0, pc1 = 0;
     if (FAST) {
       int q0 = ti * nt;
       if (M.pcount[q0] > 0) {
@@ -748,7 +761,9 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
         const bool m1 = tj == 1;
         const Real pa = m1 ? pa1 : pa0, pb = m1 ? pb1 : pb0, pc = m1 ? pc1 : pc0;
         Real c = lj_fast<Real, KM>(r2, pa, pb, pw);
-        c = (live && other && r2 <= pc) ? c : Real(0);
+        // atomic models: r2 > 0 excludes the particle itself, inside in_range
+        const bool take = MOL ? (other && r2 <= pc) : in_range(r2, pc);
+        c = (live && take) ? c : Real(0);
         fx += c * dx;
         fy += c * dy;
       } else {
