@@ -18,10 +18,12 @@ types as pyrdsim/pyrdsim.pyx.  Differences, all deliberate:
   ``.npz`` file with the same datasets.
 
 Engine options accepted in the constructor dict (not in the files, whose
-parser keeps the reference's key table): ``precision`` ('fp64' default, or
-'mixed' = FP32 pair forces with FP64 positions), ``reaction_probability``
-('linear' = the reference's p = dt*k, or 'exp' = 1 - exp(-dt*k)),
-``graph_steps`` (steps per captured hipGraph, default 32).
+parser keeps the reference's key table): ``precision`` ('fp64' default,
+'mixed' = FP32 pair forces with FP64 positions, or 'compensated' = FP32 pair
+forces with float-pair positions), ``reaction_probability`` ('linear' = the
+reference's p = dt*k, or 'exp' = 1 - exp(-dt*k)), ``graph_steps`` (steps per
+captured hipGraph, default 32), ``skin`` (0 default: cell search every step;
+> 0: Verlet lists rebuilt after a particle moved more than skin/2).
 """
 
 import os
@@ -82,6 +84,7 @@ class system:
         self.precision = 'fp64'
         self.reaction_probability = 'linear'
         self.graph_steps = 32
+        self.skin = 0.0
 
     def set_system_parameters(self):
         """pyrdsim.pyx:338-400."""
@@ -195,7 +198,7 @@ class system:
         self.rs = parameters['rs'] if 'rs' in parameters else 2 * self.max_rc
         if 'neighbor_list_update_frequency' in parameters:
             self.neighbor_list_update_frequency = parameters['neighbor_list_update_frequency']
-        for key in ('precision', 'reaction_probability', 'graph_steps'):
+        for key in ('precision', 'reaction_probability', 'graph_steps', 'skin'):
             if key in parameters:
                 setattr(self, key, parameters[key])
 
@@ -308,7 +311,7 @@ class system:
         eng = Engine([model], [rep], precision=self.precision, rng='philox',
                      reaction_prob=self.reaction_probability,
                      bonds=bonds if len(bonds[0]) else None, has_molecules=has_mol,
-                     graph_steps=int(self.graph_steps))
+                     graph_steps=int(self.graph_steps), skin=float(self.skin))
         eng.set_random_force(random_force)
         return eng
 

@@ -79,6 +79,15 @@ def test_simulate_result_and_frames(gpu, workdir, name):
     d = out2['x'] - out['x']
     d -= s['L'] * np.round(d / s['L']) * (s['periodic'] > 0)
     assert np.abs(d).max() < 1e-9
+    # engine options in the constructor dict: Verlet lists and compensated
+    # positions give the same run to rounding
+    for opts in ({'skin': 0.4}, {'precision': 'compensated'}):
+        sim3 = system(dict(seed=123, **opts))
+        out3 = sim3.simulate(verbose=False)
+        d = out3['x'] - out['x']
+        d -= s['L'] * np.round(d / s['L']) * (s['periodic'] > 0)
+        assert np.abs(d).max() < 1e-6, opts
+        assert np.array_equal(out3['particle_types'], out['particle_types']), opts
 
 
 def test_resume_with_append(gpu, workdir):
