@@ -22,8 +22,18 @@ from pyrdsim_amd.autotune import autotune
 pytestmark = pytest.mark.gpu
 
 
+def _cluster_distribution(x, t, s, kmax=64):
+    """Share of the type-1 particles in clusters of each size (linked at
+    1.5 sigma), sizes above kmax pooled."""
+    sizes = analysis.cluster_sizes(x, t, s['L'], s['periodic'], active=1, r_link=1.5)
+    h = np.zeros(kmax + 1)
+    for k in sizes:
+        h[min(int(k), kmax)] += k
+    return h / max(1.0, h.sum())
+
+
 def _averaged(eng, s, steps, every):
-    gs, acts = [], []
+    gs, acts, cls = [], [], []
     for k in range(steps // every):
         eng.run(every)
         if k >= (steps // every) // 2:
@@ -31,12 +41,16 @@ def _averaged(eng, s, steps, every):
             _, g = analysis.rdf(x, s['L'], s['periodic'])
             gs.append(g)
             acts.append(analysis.activated_fraction(t))
-    return np.mean(gs, axis=0), float(np.mean(acts))
+            cls.append(_cluster_distribution(x, t, s))
+    return np.mean(gs, axis=0), float(np.mean(acts)), np.mean(cls, axis=0)
 
 
-@pytest.mark.parametrize('name', ['kalj', 'wall'])
+@pytest.mark.parametrize('name', ['kalj', 'wall', 'patchy'])
 def test_mixed_precision_rdf_within_one_percent(gpu, name):
-    s = small(name, n=6000, seed=21)
+    """RDF within 1%, activated fraction within 0.01 and the cluster-size
+    distribution within 0.05 (L1) of the FP64 run, averaged over the second
+    half of the run."""
+    s = small(name, n_mol=1500, seed=21) if name == 'patchy' else small(name, n=6000, seed=21)
     steps, every = 4000, 200
     res = {}
     for prec in ('fp64', 'mixed', 'compensated'):
@@ -47,6 +61,50 @@ def test_mixed_precision_rdf_within_one_percent(gpu, name):
         err = analysis.rdf_error(res[prec][0], res['fp64'][0])
         assert err <= 0.01, (prec, err)
         assert abs(res[prec][1] - res['fp64'][1]) <= 0.01, prec
+        assert np.abs(res[prec][2] - res['fp64'][2]).sum() <= 0.05, prec
+
+
+@pytest.mark.parametrize('name', ['kalj', 'wall'])
+def test_full_size_configurations(gpu, name):
+    """The two single-system configurations at their full sizes (LJ mixture
+    N = 262,144 at density 0.6; wall + reactions N = 65,536 at density 0.4):
+    with the same Philox streams, FP32-force runs (FP64 or compensated
+    positions) stay within 2e-5 sigma of the FP64 run after 100 steps (the
+    seeded states hold close pairs whose forces reach 1e3), keep every
+    particle in the box, and make the same reaction events up to flips at
+    pair distances within the runs' difference of r0."""
+    from pyrdsim_amd import workloads
+    s = workloads.kalj() if name == 'kalj' else workloads.wall()
+    if name == 'wall':
+        s['types'][::5] = 1
+    out, ev = {}, {}
+    for prec in ('fp64', 'mixed', 'compensated'):
+        eng = engine_for(s, precision=prec, seed=9, graph_steps=25)
+        eng.run(100)
+        ev[prec] = eng.sync().reaction_events
+        out[prec] = eng.state()
+        eng.close()
+    L = s['L']
+    for prec in ('mixed', 'compensated'):
+        x, t = out[prec]
+        assert np.isfinite(x).all() and (x >= 0).all() and (x <= L).all()
+        # a reaction decision can flip where a pair distance lies within the
+        # runs' 1e-8 difference of r0; a flipped type changes its force law, so
+        # positions are compared away from such particles
+        flip = t != out['fp64'][1]
+        assert flip.sum() <= 20 and abs(ev[prec] - ev['fp64']) <= 20, (prec, int(flip.sum()))
+        keep = np.ones(len(x), dtype=bool)
+        if flip.any():
+            from scipy.spatial import cKDTree
+            box = np.where(s['periodic'] > 0, L, 4 * L + 100)
+            tree = cKDTree(np.mod(out['fp64'][0], L) % L, boxsize=box)
+            for nb in tree.query_ball_point(np.mod(out['fp64'][0][flip], L) % L, 4.0):
+                keep[nb] = False
+        d = x - out['fp64'][0]
+        d = d - L * np.round(d / L) * (s['periodic'] > 0)
+        print(name, prec, 'max deviation from FP64:', float(np.abs(d[keep]).max()),
+              'flipped types:', int(flip.sum()))
+        assert np.abs(d[keep]).max() < 2e-5, (prec, np.abs(d[keep]).max())
 
 
 @pytest.mark.parametrize('name', ['kalj', 'wall', 'patchy'])
