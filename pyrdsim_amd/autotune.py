@@ -13,7 +13,7 @@ signature in a small JSON file.
 import json
 import os
 
-CANDIDATES = ((1, 56), (2, 48), (2, 56), (2, 64), (2, 80), (4, 48), (4, 56), (4, 64), (4, 96),
+CANDIDATES = ((2, 56), (2, 64), (2, 96), (4, 56), (4, 64), (4, 80), (4, 96), (4, 112), (4, 128),
               (4, 144))
 
 

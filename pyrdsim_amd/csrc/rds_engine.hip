@@ -46,6 +46,7 @@
 #define MAXWALLS RDS_MAX_WALLS
 #define MAXRX RDS_MAX_REACTIONS
 #define MAXW 60        // cells per tile strip
+#define SBINS 128      // bins of the per-tile sort of home cells by stencil count
 
 namespace {
 
@@ -332,8 +333,39 @@ struct TileSh {
   double seg_sx[R + 2][3];
   double row_sy[R + 2];
   int hp[R + 1];             // home particles of the home rows before row q
+  int ckey[R * MAXW];        // home cells (q << 6 | k), by stencil count, largest first
+  int chp[R * MAXW + 1];     // home particles of the sorted cells before cell i
+  int sbin[SBINS];
   int total, nbatch;
 };
+
+// Exclusive prefix sum of a[0..n) in place, by the 64 lanes of the calling
+// wave (n <= 64 * PER); returns the total to every lane.
+template <int PER>
+__device__ __forceinline__ int wave_exclusive_scan(int* a, int n, int lane) {
+  int v[PER];
+  int sum = 0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int c = lane * PER + i;
+    v[i] = c < n ? a[c] : 0;
+    sum += v[i];
+  }
+  int inc = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  int run = inc - sum;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int c = lane * PER + i;
+    if (c < n) a[c] = run;
+    run += v[i];
+  }
+  return __shfl(inc, 63, 64);
+}
 
 template <int R>
 __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int cy0, int cx0,
@@ -423,6 +455,34 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
     nb = (hsum + 64 * R - 1) / (64 * R);
     sh.nbatch = nb;
   }
+  // Home cells in order of their stencil's candidate count, largest first
+  // (counting sort), so that the lanes of one wave get home particles with
+  // windows of similar length.
+  const int nc = nrows * W;
+  for (int q = t; q < SBINS; q += nthr) sh.sbin[q] = 0;
+  __syncthreads();
+  auto stencil = [&](int c) {
+    const int q = c / W, k = c - q * W;
+    int S = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) S += sh.off[q + d][k + 3] - sh.off[q + d][k];
+    return min(S, SBINS - 1);
+  };
+  for (int c = t; c < nc; c += nthr) atomicAdd(&sh.sbin[SBINS - 1 - stencil(c)], 1);
+  __syncthreads();
+  if (t < 64) wave_exclusive_scan<SBINS / 64>(sh.sbin, SBINS, t);
+  __syncthreads();
+  for (int c = t; c < nc; c += nthr) {
+    const int pos = atomicAdd(&sh.sbin[SBINS - 1 - stencil(c)], 1);
+    const int q = c / W, k = c - q * W;
+    sh.ckey[pos] = (q << 6) | k;
+    sh.chp[pos] = sh.off[q + 1][k + 2] - sh.off[q + 1][k + 1];
+  }
+  __syncthreads();
+  if (t < 64) {
+    const int tot = wave_exclusive_scan<(R * MAXW + 63) / 64>(sh.chp, nc, t);
+    if (t == 0) sh.chp[nc] = tot;
+  }
   __syncthreads();
 }
 
@@ -480,10 +540,10 @@ __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
 }
 
 // Home particle of lane `lane` of wave wv in batch b, and its three windows.
-// The home particles of the tile's rows, concatenated, are dealt to the waves
-// 64 at a time: home number (b * R + wv) * 64 + lane.  The home cell is the one
-// the slot was sorted into: scell (list mode, cells of the last rebuild) or,
-// with scell null, the cell of the current position.
+// The tile's home particles, taken cell by cell in the order of the sorted
+// home cells, are dealt to the waves 64 at a time: home number
+// (b * R + wv) * 64 + lane.  The home cell is the cell whose slot range holds
+// the particle (the cell it was sorted into).
 template <int R>
 struct Home {
   int s;
@@ -494,37 +554,39 @@ struct Home {
 
 template <int R>
 __device__ __forceinline__ void home_windows(const TileSh<R>& sh, int wv, int b, int lane,
-                                             int nrows, const DRep& Rp, int cx0, int W,
-                                             const double2* __restrict__ pos,
-                                             const int* __restrict__ scell, Home<R>& h,
+                                             int nrows, int W,
+                                             const double2* __restrict__ pos, Home<R>& h,
                                              double2& xi, int cpos = 0) {
   const int g = (b * R + wv) * 64 + lane;
-  int q = 0;  // home row of this lane
-#pragma unroll
-  for (int r = 1; r < R; ++r) q = (r < nrows && g >= sh.hp[r]) ? r : q;
+  const int nc = nrows * W;
+  h.valid = g < sh.chp[nc];
+  // the sorted cell holding home number g: largest i with chp[i] <= g
+  int lo = 0, hi = nc;
+  if (h.valid) {
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (sh.chp[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+  }
+  const int key = sh.ckey[lo];
+  const int q = key >> 6, k = key & 63;
   const int hr = q + 1;  // staged row of the home row
-  h.valid = g < sh.hp[nrows];
-  h.s = h.valid ? sh.cs[hr][1] + (g - sh.hp[q]) : 0;
   xi = make_double2(0.0, 0.0);
-  h.w = 0;
+  h.s = 0;
+  h.w = k;
   h.own = -1;
   if (h.valid) {
+    const int r = g - sh.chp[lo];
+    h.s = sh.cs[hr][k + 1] + r;
+    h.own = sh.row_base[hr] + sh.off[hr][k + 1] + r;
     xi = rd_pos(pos, h.s, cpos);
-    int cx;
-    if (scell) {
-      const int c = scell[h.s] - Rp.cbase;
-      cx = c - (c / Rp.lx) * Rp.lx;
-    } else {
-      cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
-    }
-    h.w = min(max(cx - cx0, 0), W - 1);
-    h.own = sh.row_base[hr] + sh.off[hr][h.w + 1] + (h.s - sh.cs[hr][h.w + 1]);
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const int r = q + d;
-    h.lo[d] = h.valid ? sh.row_base[r] + sh.off[r][h.w] : 0;
-    h.hi[d] = h.valid ? sh.row_base[r] + sh.off[r][h.w + 3] : 0;
+    const int rr = q + d;
+    h.lo[d] = h.valid ? sh.row_base[rr] + sh.off[rr][k] : 0;
+    h.hi[d] = h.valid ? sh.row_base[rr] + sh.off[rr][k + 3] : 0;
   }
 }
 
@@ -710,7 +772,7 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
   for (int b = 0; b < sh.nbatch; ++b) {
     Home<R> h;
     double2 xi;
-    home_windows<R>(sh, wv, b, lane, nrows, Rp, cx0, W, A.pos, scell, h, xi, PM);
+    home_windows<R>(sh, wv, b, lane, nrows, W, A.pos, h, xi, PM);
     int ti = 0, mi = 0, p = 0;
     // position in the frame of the staged tile (list mode: nearest image to
     // the sorted cell); xi itself stays the integrator's coordinate
@@ -829,17 +891,27 @@ __global__ __launch_bounds__(64 * R) void k_force(const StepArgs A, int mode, in
           }
         }
       }
+      {
+        // the three row windows (clipped to this LDS round) as one sequence,
+        // so that a lane's trip count is its total; two candidates per trip,
+        // past the end the second one repeats a staged index with its
+        // contribution masked (0 * dx of an unstaged one could be 0 * NaN)
+        int nn[3], oo[3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        const int lo = max(h.lo[d], p0) - p0;
-        const int hi = min(h.hi[d], p0 + CAPT) - p0;
-        // two candidates per trip; past the window end the second one
-        // repeats j with its contribution masked (it must read a staged
-        // element: 0 * dx of an unstaged one could be 0 * NaN)
-        for (int j = lo; j < hi; j += 2) {
-          pair(j, true, false);
-          const bool two = j + 1 < hi;
-          pair(two ? j + 1 : j, two, false);
+        for (int d = 0; d < 3; ++d) {
+          const int lo = max(h.lo[d], p0) - p0;
+          const int hi = min(h.hi[d], p0 + CAPT) - p0;
+          nn[d] = max(hi - lo, 0);
+          oo[d] = lo;
+        }
+        const int b1 = nn[0], b2 = nn[0] + nn[1], n = b2 + nn[2];
+        const int o0 = oo[0], o1 = oo[1] - b1, o2 = oo[2] - b2;
+        for (int u = 0; u < n; u += 2) {
+          const int u1 = min(u + 1, n - 1);
+          const int j0 = u + (u < b1 ? o0 : (u < b2 ? o1 : o2));
+          const int j1 = u1 + (u1 < b1 ? o0 : (u1 < b2 ? o1 : o2));
+          pair(j0, true, false);
+          pair(j1, u + 1 < n, false);
         }
       }
     }
@@ -1016,7 +1088,7 @@ __global__ __launch_bounds__(64 * R) void k_build(const StepArgs A) {
     for (int b = 0; b < sh.nbatch; ++b) {
       Home<R> h;
       double2 xi;
-      home_windows<R>(sh, wv, b, lane, nrows, Rp, cx0, W, A.pos, nullptr, h, xi);
+      home_windows<R>(sh, wv, b, lane, nrows, W, A.pos, h, xi, A.cpos);
       const int mi = (MOL && h.valid) ? A.mol[h.s] : 0;
       uint2* out = A.nl4 + (h.valid ? h.s : 0);
       int n = 0;
@@ -1106,7 +1178,7 @@ __global__ __launch_bounds__(64) void k_count(const StepArgs A) {
     for (int b = 0; b < sh.nbatch; ++b) {
       Home<1> h;
       double2 xi;
-      home_windows<1>(sh, 0, b, lane, 1, Rp, cx0, W, A.pos, nullptr, h, xi, A.cpos);
+      home_windows<1>(sh, 0, b, lane, 1, W, A.pos, h, xi, A.cpos);
       int mi = (MOL && h.valid) ? A.mol[h.s] : 0;
       int cnt = 0;
       for (int p0 = 0; p0 < total; p0 += CAPT) {
