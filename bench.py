@@ -307,8 +307,10 @@ def main():
     print('# built %d replica states in %.1f s' % (len(plan), time.perf_counter() - t0),
           file=sys.stderr, flush=True)
     traffic = None
-    traffic_tiling = (args.tile_rows if args.tile_rows > 1 else 2,
-                      args.tile_particles if args.tile_particles > 0 else 56)
+    # the counter passes use the tile shape the autotuner picks for this
+    # workload (4 rows x 96 particles), unless one is given
+    traffic_tiling = (args.tile_rows if args.tile_rows > 1 else 4,
+                      args.tile_particles if args.tile_particles > 0 else 96)
     if args.traffic:
         traffic = measure_traffic(cache, traffic_tiling[0], traffic_tiling[1], args.replicas)
         print('# traffic: %s' % traffic, file=sys.stderr, flush=True)
@@ -376,6 +378,7 @@ def main():
 
     accuracy = None
     compensated = None
+    batched_fp64 = None
     if not args.no_accuracy:
         # throughput of the same batched run with compensated FP32 positions
         ec = Engine(models, reps, precision='compensated', graph_steps=args.graph_steps,
@@ -385,6 +388,16 @@ def main():
         compensated = dict(value=n_tot * args.steps / secs_c, unit='particle-steps/s',
                            ms_per_step=1e3 * secs_c / args.steps)
         del ec
+        torch.cuda.synchronize()
+        # the same batched run in FP64: separates the gain of batching from
+        # that of FP32 pair arithmetic
+        e64 = Engine(models, reps, precision='fp64', graph_steps=args.graph_steps,
+                     tile_rows=tiling[0], tile_particles=tiling[1])
+        e64.set_state(x0, t0s)
+        secs_64 = time_engine(e64, args.steps, args.warmup)
+        batched_fp64 = dict(value=n_tot * args.steps / secs_64, unit='particle-steps/s',
+                            ms_per_step=1e3 * secs_64 / args.steps)
+        del e64
         torch.cuda.synchronize()
         accuracy = accuracy_leg(plan, states, models, reps, x0, t0s, args.graph_steps, tiling,
                                 args.accuracy_steps, args.accuracy_replicas)
@@ -451,6 +464,7 @@ def main():
         'batched_vs_plain': (value / plain['value']) if plain else None,
         'accuracy': accuracy,
         'compensated_positions': compensated,
+        'batched_fp64': batched_fp64,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))
