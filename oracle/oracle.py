@@ -76,6 +76,7 @@ def load():
         'orc_destroy': (None, [vp]),
         'orc_set_injected': (C.c_int, [vp, vp, vp, vp, C.c_int64, C.c_int64]),
         'orc_init': (C.c_int, [vp]),
+        'orc_set_angles': (C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp, C.c_double]),
         'orc_run': (C.c_int, [vp, C.c_int64, C.c_int64]),
         'orc_get_state': (C.c_int, [vp, vp, vp, vp]),
         'orc_set_state': (C.c_int, [vp, vp, vp]),
@@ -133,7 +134,7 @@ class Oracle:
     def __init__(self, x0, types0, L, periodic, dt, tau_by_tauD, interactions,
                  reactions=(), molecules=None, bonds=(), seed=0, rng='philox',
                  nl='exact', rs=None, ratio=2, nl_freq=10, include_random_force=1.0,
-                 reaction_prob='linear'):
+                 reaction_prob='linear', angles=(), angle_threshold=0.1):
         lib = load()
         x0 = np.ascontiguousarray(x0, dtype=np.float64)
         n, d = x0.shape
@@ -214,6 +215,16 @@ class Oracle:
         self.h = lib.orc_create(C.byref(s), x0.ctypes.data, types0.ctypes.data)
         if not self.h:
             raise RuntimeError(lib.orc_last_error().decode())
+        # angles: (i1, i2, i3, theta0 [rad], k0 [per rad^2]) in definition order
+        if len(angles):
+            ang = list(angles)
+            self._ang = (np.ascontiguousarray([a[0] for a in ang], dtype=np.int32),
+                         np.ascontiguousarray([a[1] for a in ang], dtype=np.int32),
+                         np.ascontiguousarray([a[2] for a in ang], dtype=np.int32),
+                         np.ascontiguousarray([a[3] for a in ang], dtype=np.float64),
+                         np.ascontiguousarray([a[4] for a in ang], dtype=np.float64))
+            lib.orc_set_angles(self.h, len(ang), *[v.ctypes.data for v in self._ang],
+                               float(angle_threshold))
         self._inj = None
         self.step = 0
 

@@ -180,6 +180,17 @@ struct Oracle {
   std::vector<std::vector<double>> nl_d, nl_hat;  // per i, parallel to nl[i]
   // last computed distance/unit vector for each bond (stale when unlisted)
   std::vector<double> bond_d, bond_hat;
+  // angles (pyx:1322-1407): participants, theta0 [rad], k0 = u0 [per rad^2]
+  struct Angle {
+    int32_t i1, i2, i3;
+    double theta0, k0;
+  };
+  std::vector<Angle> angles;
+  double angle_threshold = 0.1;  // angle_pi_approximation_threshold (pyx:354)
+  // angle_is_pi as the angle closures read it: the enclosing variable at call
+  // time, i.e. the value set for the last angle defined (pyx:1334-1338, 1378)
+  double angle_is_pi = -1.0;
+  bool angle_error = false;
   // grid
   std::vector<double> cl;
   std::vector<int32_t> ncell, pbc_nc;
@@ -441,6 +452,51 @@ struct Oracle {
         a[(size_t)bonds[b].i2 * d + k] += -f * h;
       }
     }
+    // pyx:2052-2065 with the closure of pyx:1340-1391
+    for (const Angle& g : angles) {
+      double v1[3], v2[3], d1, d2;
+      pair_vec(g.i2, g.i1, v1, &d1);  // x[i1] - x[i2], minimum image
+      pair_vec(g.i2, g.i3, v2, &d2);  // x[i3] - x[i2]
+      double u1[3], u2[3], h = 0.0;
+      for (int k = 0; k < d; ++k) {
+        u1[k] = v1[k] / d1;
+        u2[k] = v2[k] / d2;
+        h += u1[k] * u2[k];
+      }
+      double theta;
+      if (h < -1.) {
+        if (h < -1.0000001) {
+          angle_error = true;
+          char buf[160];
+          snprintf(buf, sizeof buf,
+                   "Inner product of units vectors is smaller than -1. Value = %.17g", h);
+          g_err = buf;
+        }
+        h = -1.;
+        theta = M_PI;
+      } else {
+        theta = std::acos(h);
+      }
+      double prefac;
+      if (angle_is_pi > 0.) {
+        if (theta < g.theta0 - angle_threshold) {
+          prefac = g.k0 * (theta - g.theta0) / std::sqrt(1. - h * h);
+        } else {
+          double t = theta - g.theta0;
+          prefac = -g.k0 / (1. - t * t / 6. + (t * t * t * t) / 120.);
+        }
+      } else {
+        prefac = g.k0 * (theta - g.theta0) / std::sqrt(1. - h * h);
+      }
+      for (int k = 0; k < d; ++k) {
+        double f = prefac / d1 * (u2[k] - h * u1[k]);
+        a[(size_t)g.i1 * d + k] += f;
+        a[(size_t)g.i2 * d + k] -= f;
+        f = prefac / d2 * (u1[k] - h * u2[k]);
+        a[(size_t)g.i3 * d + k] += f;
+        a[(size_t)g.i2 * d + k] -= f;
+      }
+    }
   }
 
   // pyx:2070-2222
@@ -553,7 +609,7 @@ struct Oracle {
       calc_distances();
     reaction_step(stepno);
     update_drifts();
-    return true;
+    return !angle_error;
   }
 };
 
@@ -622,9 +678,35 @@ int orc_init(void* h) {
   return 0;
 }
 
+// Angles (pyx:1322-1407), in definition order: participants, theta0 [rad],
+// k0 = u0 [per rad^2], and the pi-approximation threshold (pyx:354).  Their
+// legs join the bonded neighbour list (pyx:1405-1406).  Call before orc_init.
+int orc_set_angles(void* h, int64_t n, const int32_t* i1, const int32_t* i2,
+                   const int32_t* i3, const double* theta0, const double* k0,
+                   double threshold) {
+  Oracle* o = (Oracle*)h;
+  o->angles.clear();
+  for (int64_t q = 0; q < n; ++q) {
+    o->angles.push_back(Oracle::Angle{i1[q], i2[q], i3[q], theta0[q], k0[q]});
+    auto leg = [&](int32_t p, int32_t r) {
+      int32_t lo = std::min(p, r), hi = std::max(p, r);
+      o->bonded_set[lo].insert(hi);
+      o->bonded_static[lo].insert(hi);
+    };
+    leg(i1[q], i2[q]);
+    leg(i2[q], i3[q]);
+  }
+  o->angle_threshold = threshold;
+  // np.isclose(theta0, pi, atol=1e-5): |a - b| <= atol + rtol |b|, rtol = 1e-5
+  if (n > 0)
+    o->angle_is_pi = std::fabs(theta0[n - 1] - M_PI) <= 1e-5 + 1e-5 * M_PI ? 1.0 : -1.0;
+  return 0;
+}
+
 // Run integration steps first_step .. first_step+n_steps-1 (the reference's
 // step argument l+1).  Returns 0, or -3 on an out-of-bounds coordinate
-// (message in orc_last_error), -1 on an exhausted injected stream.
+// (message in orc_last_error) or an angle whose cosine is below -1.0000001
+// (pyx:1364-1370), -1 on an exhausted injected stream.
 int orc_run(void* h, int64_t first_step, int64_t n_steps) {
   Oracle* o = (Oracle*)h;
   try {

@@ -240,3 +240,71 @@ def test_two_state_kinetics():
     o.run(1, 1000)
     frac = (o.state()[1] == 1).mean()
     assert abs(frac - ka / (ka + kd)) < 0.02
+
+
+def _trimer_drift(x, angles, L=50.0):
+    """Oracle drift of three particles with only angle terms."""
+    from oracle import Oracle
+    x = np.asarray(x, dtype=np.float64)
+    o = Oracle(x, np.zeros(len(x), dtype=np.int32), [L, L], [1, 1], 1e-4, [1.0], [],
+               molecules=np.zeros(len(x), dtype=np.int32), angles=angles)
+    o.init()
+    return o.state()[2]
+
+
+def _angle_energy(x, i1, i2, i3, theta0, k0):
+    u = x[i1] - x[i2]
+    v = x[i3] - x[i2]
+    h = u @ v / np.sqrt(u @ u) / np.sqrt(v @ v)
+    return 0.5 * k0 * (np.arccos(np.clip(h, -1, 1)) - theta0) ** 2
+
+
+@pytest.mark.parametrize('theta0,bend', [(2 * np.pi / 3, 0.3), (np.pi, 0.2)])
+def test_angle_force_is_minus_gradient(theta0, bend):
+    """The angle force (pyx:2052-2065) is -dU/dx of U = (k0/2)(theta - theta0)^2
+    (the GROMACS harmonic angle), for an ordinary angle and, on the series
+    branch, for a straight one (theta0 = pi)."""
+    k0 = 7.0
+    phi = theta0 - bend
+    x = np.array([[10.0 + np.cos(0.4), 10.0 + np.sin(0.4)], [10.0, 10.0],
+                  [10.0 + 1.3 * np.cos(0.4 + phi), 10.0 + 1.3 * np.sin(0.4 + phi)]])
+    a = _trimer_drift(x, [(0, 1, 2, theta0, k0)])
+    g = np.zeros_like(x)
+    h = 1e-6
+    for p in range(3):
+        for k in range(2):
+            xp, xm = x.copy(), x.copy()
+            xp[p, k] += h
+            xm[p, k] -= h
+            g[p, k] = (_angle_energy(xp, 0, 1, 2, theta0, k0) -
+                       _angle_energy(xm, 0, 1, 2, theta0, k0)) / (2 * h)
+    assert np.abs(a + g).max() < 1e-6 * (np.abs(g).max() + 1), (a, -g)
+    assert np.abs(a.sum(axis=0)).max() < 1e-12
+
+
+def test_angle_pi_flag_binds_late():
+    """The angle closures read angle_is_pi when they are called, not when they
+    are made (pyx:1334-1338 vs 1378): every angle uses the value set for the
+    last angle defined.  A 120-degree angle followed by a straight one is
+    therefore evaluated with the series formula of pyx:1385-1386 when
+    theta >= theta0 - 0.1."""
+    k0, theta0 = 7.0, 2 * np.pi / 3
+    phi = theta0 + 0.05
+    tri = np.array([[10.0 + np.cos(0.4), 10.0 + np.sin(0.4)], [10.0, 10.0],
+                    [10.0 + 1.3 * np.cos(0.4 + phi), 10.0 + 1.3 * np.sin(0.4 + phi)]])
+    line = np.array([[20.0, 20.0], [21.0, 20.0], [22.0, 20.02]])
+    x = np.concatenate([tri, line])
+    straight = (3, 4, 5, np.pi, k0)
+    bent = (0, 1, 2, theta0, k0)
+    a_last = _trimer_drift(x, [bent, straight])   # flag of the straight angle
+    a_first = _trimer_drift(x, [straight, bent])  # flag of the bent angle
+    u, v = tri[0] - tri[1], tri[2] - tri[1]
+    d1, d2 = np.linalg.norm(u), np.linalg.norm(v)
+    u, v = u / d1, v / d2
+    hh = u @ v
+    t = np.arccos(hh) - theta0
+    series = -k0 / (1 - t ** 2 / 6 + t ** 4 / 120)
+    exact = k0 * t / np.sqrt(1 - hh ** 2)
+    assert np.allclose(a_last[0], series / d1 * (v - hh * u), rtol=1e-12, atol=1e-12)
+    assert np.allclose(a_first[0], exact / d1 * (v - hh * u), rtol=1e-12, atol=1e-12)
+    assert not np.allclose(series, exact)
