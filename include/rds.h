@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define RDS_ABI_VERSION 2
+#define RDS_ABI_VERSION 3
 
 #define RDS_MAX_TYPES 8
 #define RDS_MAX_TERMS 32
@@ -41,7 +41,8 @@ enum {
   RDS_E_HIP = -2,      /* HIP runtime error */
   RDS_E_BOUNDS = -3,   /* a coordinate left the box (pyrdsim.pyx:2286-2292) */
   RDS_E_STATE = -4,    /* call out of order (e.g. run before set_state) */
-  RDS_E_STREAM = -5    /* injected random stream exhausted */
+  RDS_E_STREAM = -5,   /* injected random stream exhausted */
+  RDS_E_ANGLE = -6     /* an angle's cosine below -1.0000001 (pyrdsim.pyx:1364-1370) */
 };
 
 /* Precision.  RDS_FP64: FP64 pair forces and positions.  RDS_MIXED: FP32
@@ -144,6 +145,24 @@ typedef struct {
   double skin;
   int32_t list_capacity;
   int32_t reserved0;
+  /* harmonic angles (pyrdsim.pyx:1322-1407, 2052-2065) in definition order,
+   * global particle indices, host arrays; n_angles may be 0.  angle_j is the
+   * vertex.  theta0 in rad and k0 per rad^2, as the reference holds them
+   * after parsing (pyrdsim.pyx:987-988: theta0 = pi deg/180,
+   * k0 = k (180/pi)^2), so that U = (k0/2)(theta - theta0)^2.  Every angle
+   * takes the straight-angle series (pyrdsim.pyx:1378-1386) when the LAST
+   * angle defined has theta0 close to pi (np.isclose, atol 1e-5), because the
+   * reference's angle closures read that flag when they are called.
+   * angle_threshold is the reference's angle_pi_approximation_threshold
+   * (0.1, pyrdsim.pyx:354).  The legs (i, j) and (j, k) count as bonded
+   * partners of the catalysis search, like bonds (pyrdsim.pyx:1405-1406). */
+  int64_t n_angles;
+  const int32_t* angle_i;
+  const int32_t* angle_j;
+  const int32_t* angle_k;
+  const double* angle_theta0;
+  const double* angle_k0;
+  double angle_threshold;
 } rds_config;
 
 /* Result of the last rds_sync. */
@@ -157,7 +176,8 @@ typedef struct {
   double error_value;       /* coordinate after the boundary conditions */
   int64_t reaction_events;  /* type changes since set_state */
   int32_t integrity;        /* nonzero: an internal cell-list invariant failed */
-  int32_t reserved;
+  int32_t angle_error;      /* 1 if an angle's cosine fell below -1.0000001
+                               (error_value holds it when bounds_error is 0) */
   int64_t rebuilds;         /* list mode: sorts + list builds since set_state */
   int64_t list_overflow;    /* list mode: particles that exceeded list_capacity */
   int64_t list_entries;     /* list mode: sum of list lengths over all builds */
@@ -199,12 +219,12 @@ int rds_set_random_force(rds_ctx* ctx, int32_t include);
  * update_drifts).  Asynchronous. */
 int rds_run(rds_ctx* ctx, int64_t n_steps);
 
-/* Waits for the stream and reports the status; returns RDS_E_BOUNDS /
- * RDS_E_STREAM when the status holds such an error. */
+/* Waits for the stream and reports the status; returns RDS_E_BOUNDS,
+ * RDS_E_ANGLE or RDS_E_STREAM when the status holds such an error. */
 int rds_sync(rds_ctx* ctx, rds_status* status);
 
 /* Drift of the current state, [N][2] f64 in caller order (update_drifts,
- * pyrdsim.pyx:1941-2067: pair forces, walls and bonds). */
+ * pyrdsim.pyx:1941-2067: pair forces, walls, bonds and angles). */
 int rds_compute_forces(rds_ctx* ctx, double* forces);
 
 /* Cell of each particle (global cell index) in caller order, and the grid:

@@ -14,9 +14,9 @@ MAX_TYPES = 8
 MAX_TERMS = 32
 MAX_WALLS = 8
 MAX_REACTIONS = 8
-ABI_VERSION = 2
+ABI_VERSION = 3
 
-OK, E_ARG, E_HIP, E_BOUNDS, E_STATE, E_STREAM = 0, -1, -2, -3, -4, -5
+OK, E_ARG, E_HIP, E_BOUNDS, E_STATE, E_STREAM, E_ANGLE = 0, -1, -2, -3, -4, -5, -6
 FP64, MIXED, COMPENSATED = 0, 1, 2
 RNG_PHILOX, RNG_INJECTED = 0, 1
 PROB_LINEAR, PROB_EXP = 0, 1
@@ -76,7 +76,11 @@ class Config(C.Structure):
                 ('n_bonds', C.c_int64), ('bond_i', C.POINTER(C.c_int32)),
                 ('bond_j', C.POINTER(C.c_int32)), ('bond_r0', C.POINTER(C.c_double)),
                 ('bond_u0', C.POINTER(C.c_double)), ('skin', C.c_double),
-                ('list_capacity', C.c_int32), ('reserved0', C.c_int32)]
+                ('list_capacity', C.c_int32), ('reserved0', C.c_int32),
+                ('n_angles', C.c_int64), ('angle_i', C.POINTER(C.c_int32)),
+                ('angle_j', C.POINTER(C.c_int32)), ('angle_k', C.POINTER(C.c_int32)),
+                ('angle_theta0', C.POINTER(C.c_double)),
+                ('angle_k0', C.POINTER(C.c_double)), ('angle_threshold', C.c_double)]
 
 
 class Status(C.Structure):
@@ -84,7 +88,7 @@ class Status(C.Structure):
                 ('stream_error', C.c_int32), ('error_step', C.c_int64),
                 ('error_particle', C.c_int32), ('error_axis', C.c_int32),
                 ('error_value', C.c_double), ('reaction_events', C.c_int64),
-                ('integrity', C.c_int32), ('reserved', C.c_int32),
+                ('integrity', C.c_int32), ('angle_error', C.c_int32),
                 ('rebuilds', C.c_int64), ('list_overflow', C.c_int64),
                 ('list_entries', C.c_int64)]
 

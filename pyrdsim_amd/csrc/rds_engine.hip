@@ -10,7 +10,7 @@
 // positions as double2, int32 type / pid / molecule columns.  Many independent
 // systems ("replicas") are concatenated: replica r owns slots
 // [pbase_r, pbase_r + N_r) and cells [cbase_r, cbase_r + lx_r*ly_r).  Per step:
-//   k_bonds    (molecular models only) harmonic bond forces per slot
+//   k_bonds    (molecular models only) harmonic bond and angle forces per slot
 //   k_force    one wavefront per tile (a strip of W cells of one cell row): the
 //              3 x (W+2) neighbour cells are staged in LDS with their periodic
 //              images already shifted; each lane owns one home particle and
@@ -124,7 +124,8 @@ struct DStatus {
   int rebuild;                 // list mode: set when a particle moved more than skin/2
   unsigned long long rebuilds, overflow, entries;
   unsigned done;               // blocks of the list build that finished (reset by the last)
-  int pad;
+  int angle;                   // an angle's cosine fell below -1.0000001
+  double angle_val;            // that cosine
 };
 
 #define LIST_OVER 0xFFFF  // list length marking a particle that uses the stencil scan
@@ -1374,31 +1375,108 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
 }
 
 // ---------------------------------------------------------------------------
-// harmonic bonds (pyrdsim.pyx:1303-1320, 2037-2047), FP64, per slot
+// harmonic bonds (pyrdsim.pyx:1303-1320, 2037-2047) and angles
+// (pyrdsim.pyx:1322-1407, 2052-2065), FP64, per slot
 // ---------------------------------------------------------------------------
 
-__global__ void k_bonds(int n_total, const DRep* __restrict__ rep, const int* __restrict__ rep_of,
-                        const double2* __restrict__ pos, const int* __restrict__ pid,
-                        const int* __restrict__ slot_of, const int* __restrict__ boff,
-                        const int* __restrict__ bpart, const double* __restrict__ br0,
-                        const double* __restrict__ bu0, double2* __restrict__ fbond, int cpos) {
+struct BondArgs {
+  int n_total;
+  const DRep* rep;
+  const int* rep_of;
+  const double2* pos;
+  const int* pid;
+  const int* slot_of;
+  int cpos;
+  // bond CSR by pid, both directions; entries with u0 = 0 are angle legs,
+  // listed only so that the reaction kernel sees them as bonded partners
+  const int* boff;
+  const int* bpart;
+  const double* br0;
+  const double* bu0;
+  // angle CSR by pid: (angle, role 0/1/2 = first/vertex/last) per membership
+  const int* aoff;
+  const int2* aent;
+  const int4* apart;    // (i1, i2 vertex, i3, -)
+  const double2* aprm;  // (theta0 [rad], k0 [per rad^2])
+  int angle_pi;         // the straight-angle flag of the last angle defined
+  double angle_thr;     // angle_pi_approximation_threshold
+  DStatus* status;
+  double2* fbond;
+};
+
+__device__ __forceinline__ double2 bond_vec(const BondArgs& A, const DRep& R, double2 from,
+                                            int to_pid) {
+  const double2 x = rd_pos(A.pos, A.slot_of[to_pid], A.cpos);
+  double dx = x.x - from.x, dy = x.y - from.y;
+  if (R.px) dx = min_image(dx, R.L[0]);
+  if (R.py) dy = min_image(dy, R.L[1]);
+  return make_double2(dx, dy);
+}
+
+__global__ void k_bonds(BondArgs A) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_total) return;
-  int p = pid[s];
-  const DRep& R = rep[rep_of[p]];
-  double2 xi = rd_pos(pos, s, cpos);
+  if (s >= A.n_total) return;
+  int p = A.pid[s];
+  const DRep& R = A.rep[A.rep_of[p]];
+  double2 xi = rd_pos(A.pos, s, A.cpos);
   double fx = 0.0, fy = 0.0;
-  for (int b = boff[p]; b < boff[p + 1]; ++b) {
-    double2 xj = rd_pos(pos, slot_of[bpart[b]], cpos);
-    double dx = xj.x - xi.x, dy = xj.y - xi.y;
-    if (R.px) dx = min_image(dx, R.L[0]);
-    if (R.py) dy = min_image(dy, R.L[1]);
-    double d = sqrt(dx * dx + dy * dy);
-    double f = bu0[b] * (d - br0[b]);
-    fx += f * (dx / d);
-    fy += f * (dy / d);
+  for (int b = A.boff[p]; b < A.boff[p + 1]; ++b) {
+    if (A.bu0[b] == 0.0) continue;
+    const double2 v = bond_vec(A, R, xi, A.bpart[b]);
+    double d = sqrt(v.x * v.x + v.y * v.y);
+    double f = A.bu0[b] * (d - A.br0[b]);
+    fx += f * (v.x / d);
+    fy += f * (v.y / d);
   }
-  fbond[s] = make_double2(fx, fy);
+  if (A.aoff) {
+    for (int e = A.aoff[p]; e < A.aoff[p + 1]; ++e) {
+      const int2 en = A.aent[e];
+      const int4 g = A.apart[en.x];
+      const double2 prm = A.aprm[en.x];
+      // v1 = x[i1] - x[i2], v2 = x[i3] - x[i2] (minimum image), the same
+      // arithmetic in all three memberships so that they see one angle
+      const double2 xv = rd_pos(A.pos, A.slot_of[g.y], A.cpos);
+      const double2 v1 = bond_vec(A, R, xv, g.x);
+      const double2 v2 = bond_vec(A, R, xv, g.z);
+      const double d1 = sqrt(v1.x * v1.x + v1.y * v1.y);
+      const double d2 = sqrt(v2.x * v2.x + v2.y * v2.y);
+      const double u1x = v1.x / d1, u1y = v1.y / d1;
+      const double u2x = v2.x / d2, u2y = v2.y / d2;
+      double h = u1x * u2x + u1y * u2y;
+      double theta;
+      if (h < -1.0) {
+        // below -1.0000001 the reference raises; rounding above it is
+        // taken as a straight angle
+        if (h < -1.0000001 && atomicExch(&A.status->angle, 1) == 0) A.status->angle_val = h;
+        h = -1.0;
+        theta = M_PI;
+      } else {
+        theta = acos(h);
+      }
+      const double t = theta - prm.x;
+      double prefac;
+      if (A.angle_pi && !(theta < prm.x - A.angle_thr)) {
+        // sin(t)/t by its series: theta0 = pi makes 1 - h^2 vanish
+        prefac = -prm.y / (1.0 - t * t / 6.0 + (t * t * t * t) / 120.0);
+      } else {
+        prefac = prm.y * t / sqrt(1.0 - h * h);
+      }
+      const double c1 = prefac / d1, c3 = prefac / d2;
+      const double f1x = c1 * (u2x - h * u1x), f1y = c1 * (u2y - h * u1y);
+      const double f3x = c3 * (u1x - h * u2x), f3y = c3 * (u1y - h * u2y);
+      if (en.y == 0) {
+        fx += f1x;
+        fy += f1y;
+      } else if (en.y == 2) {
+        fx += f3x;
+        fy += f3y;
+      } else {
+        fx -= f1x + f3x;
+        fy -= f1y + f3y;
+      }
+    }
+  }
+  A.fbond[s] = make_double2(fx, fy);
 }
 
 // ---------------------------------------------------------------------------
@@ -1657,6 +1735,13 @@ struct rds_ctx {
   double* bu0 = nullptr;
   double2* fbond = nullptr;
   int* slot_of = nullptr;
+  // angles
+  int* aoff = nullptr;
+  int2* aent = nullptr;
+  int4* apart = nullptr;
+  double2* aprm = nullptr;
+  int angle_pi = 0;
+  double angle_thr = 0.1;
   // injected streams
   const double* inj_n = nullptr;
   const double* inj_u1 = nullptr;
@@ -1841,6 +1926,31 @@ struct Timer {
   bool on = false;
 };
 
+// bond and angle forces of the current state into fbond
+void launch_bonds(rds_ctx* c, const Buf& cur) {
+  BondArgs b;
+  b.n_total = (int)c->n_total;
+  b.rep = c->d_rep;
+  b.rep_of = c->d_rep_of;
+  b.pos = cur.pos;
+  b.pid = cur.pid;
+  b.slot_of = c->slot_of;
+  b.cpos = c->cpos;
+  b.boff = c->boff;
+  b.bpart = c->bpart;
+  b.br0 = c->br0;
+  b.bu0 = c->bu0;
+  b.aoff = c->aoff;
+  b.aent = c->aent;
+  b.apart = c->apart;
+  b.aprm = c->aprm;
+  b.angle_pi = c->angle_pi;
+  b.angle_thr = c->angle_thr;
+  b.status = c->d_status;
+  b.fbond = c->fbond;
+  hipLaunchKernelGGL(k_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream, b);
+}
+
 // One integration step on the context's stream.  `ev` (optional) receives event
 // pairs around each launch.
 void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>>* ev) {
@@ -1860,9 +1970,7 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
   };
   if (c->any_bonds) {
     mark(KT_BONDS, true);
-    hipLaunchKernelGGL(k_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
-                       (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, c->slot_of,
-                       c->boff, c->bpart, c->br0, c->bu0, c->fbond, c->cpos);
+    launch_bonds(c, cur);
     mark(KT_BONDS, false);
   }
   StepArgs a = make_args(c, cur);
@@ -2140,7 +2248,7 @@ int rds_ctx_destroy(rds_ctx* c) {
   std::vector<void*> ptrs = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->d_sblocks,
                              c->upos, c->ucell, c->urank, c->count, c->cstart, c->d_step,
                              c->d_status, c->boff, c->bpart, c->br0, c->bu0, c->fbond,
-                             c->slot_of, c->nl4, c->ncnt, c->xref, c->scell, c->spos,
+                             c->slot_of, c->aoff, c->aent, c->apart, c->aprm, c->nl4, c->ncnt, c->xref, c->scell, c->spos,
                              c->stype, c->spid, c->smol};
   for (int k = 0; k < 2; ++k) {
     void* b[] = {c->buf[k].pos, c->buf[k].type, c->buf[k].type0, c->buf[k].pid, c->buf[k].mol};
@@ -2156,6 +2264,38 @@ int rds_ctx_destroy(rds_ctx* c) {
   return RDS_OK;
 }
 
+// Bonds and angles name distinct particles of one replica.
+static int check_topology(const rds_config* cfg) {
+  if (cfg->n_bonds < 0 || cfg->n_angles < 0) return fail(RDS_E_ARG, "negative bond or angle count");
+  if (cfg->n_bonds > 0 && (!cfg->bond_i || !cfg->bond_j || !cfg->bond_r0 || !cfg->bond_u0))
+    return fail(RDS_E_ARG, "bond arrays missing");
+  if (cfg->n_angles > 0 && (!cfg->angle_i || !cfg->angle_j || !cfg->angle_k ||
+                            !cfg->angle_theta0 || !cfg->angle_k0))
+    return fail(RDS_E_ARG, "angle arrays missing");
+  if (cfg->n_bonds == 0 && cfg->n_angles == 0) return RDS_OK;
+  std::vector<int64_t> ends(cfg->n_replicas);  // exclusive end pid of each replica
+  int64_t n = 0;
+  for (int r = 0; r < cfg->n_replicas; ++r) ends[r] = n += std::max(cfg->replicas[r].n_particles, 0);
+  auto rep = [&](int64_t i) {
+    return (int)(std::upper_bound(ends.begin(), ends.end(), i) - ends.begin());
+  };
+  auto valid = [&](int64_t i) { return i >= 0 && i < n; };
+  for (int64_t b = 0; b < cfg->n_bonds; ++b) {
+    const int i = cfg->bond_i[b], j = cfg->bond_j[b];
+    if (!valid(i) || !valid(j) || i == j)
+      return fail(RDS_E_ARG, "bond %lld has invalid particles (%d, %d)", (long long)b, i, j);
+    if (rep(i) != rep(j)) return fail(RDS_E_ARG, "bond %lld spans replicas", (long long)b);
+  }
+  for (int64_t q = 0; q < cfg->n_angles; ++q) {
+    const int i = cfg->angle_i[q], j = cfg->angle_j[q], k = cfg->angle_k[q];
+    if (!valid(i) || !valid(j) || !valid(k) || i == j || j == k || i == k)
+      return fail(RDS_E_ARG, "angle %lld has invalid particles (%d, %d, %d)", (long long)q, i, j, k);
+    if (rep(i) != rep(j) || rep(k) != rep(j))
+      return fail(RDS_E_ARG, "angle %lld spans replicas", (long long)q);
+  }
+  return RDS_OK;
+}
+
 int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   if (!cfg || !out) return fail(RDS_E_ARG, "null argument");
   *out = nullptr;
@@ -2168,6 +2308,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     return fail(RDS_E_ARG, "compensated FP32 positions run with skin = 0 (cell search every step)");
   if (cfg->rng != RDS_RNG_PHILOX && cfg->rng != RDS_RNG_INJECTED)
     return fail(RDS_E_ARG, "unknown rng %d", cfg->rng);
+  if (int e = check_topology(cfg)) return e;
   rds_ctx* c = new rds_ctx();
   int rc = RDS_OK;
   auto bail = [&](int code) {
@@ -2369,31 +2510,39 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     HIP_TRY(hipMemcpy(c->cstart + C, &nt, sizeof(int), hipMemcpyHostToDevice));
   }
 
-  // bonds: CSR by particle, both directions
-  if (cfg->n_bonds > 0) {
-    if (!cfg->bond_i || !cfg->bond_j || !cfg->bond_r0 || !cfg->bond_u0)
-      return bail(fail(RDS_E_ARG, "bond arrays missing"));
+  // bonds: CSR by particle, both directions.  The legs (i1, i2) and (i2, i3)
+  // of each angle join it with u0 = 0: the bond kernel skips them, and the
+  // reaction kernel counts them as bonded partners (pyrdsim.pyx:1405-1406).
+  // (indices checked by check_topology)
+  if (cfg->n_bonds > 0 || cfg->n_angles > 0) {
+    struct Leg {
+      int i, j;
+      double r0, u0;
+    };
+    std::vector<Leg> legs;
+    for (int64_t b = 0; b < cfg->n_bonds; ++b)
+      legs.push_back({cfg->bond_i[b], cfg->bond_j[b], cfg->bond_r0[b], cfg->bond_u0[b]});
+    for (int64_t q = 0; q < cfg->n_angles; ++q) {
+      legs.push_back({cfg->angle_i[q], cfg->angle_j[q], 0.0, 0.0});
+      legs.push_back({cfg->angle_j[q], cfg->angle_k[q], 0.0, 0.0});
+    }
     std::vector<int> deg(N + 1, 0);
-    for (int64_t b = 0; b < cfg->n_bonds; ++b) {
-      int i = cfg->bond_i[b], j = cfg->bond_j[b];
-      if (i < 0 || j < 0 || (size_t)i >= N || (size_t)j >= N || i == j)
-        return bail(fail(RDS_E_ARG, "bond %lld has invalid particles (%d, %d)", (long long)b, i, j));
-      deg[i + 1]++;
-      deg[j + 1]++;
+    for (const Leg& l : legs) {
+      deg[l.i + 1]++;
+      deg[l.j + 1]++;
     }
     for (size_t i = 0; i < N; ++i) deg[i + 1] += deg[i];
     std::vector<int> part(deg[N]), fill(deg.begin(), deg.end() - 1);
     std::vector<double> r0v(deg[N]), u0v(deg[N]);
-    for (int64_t b = 0; b < cfg->n_bonds; ++b) {
-      int i = cfg->bond_i[b], j = cfg->bond_j[b];
-      int q = fill[i]++;
-      part[q] = j;
-      r0v[q] = cfg->bond_r0[b];
-      u0v[q] = cfg->bond_u0[b];
-      q = fill[j]++;
-      part[q] = i;
-      r0v[q] = cfg->bond_r0[b];
-      u0v[q] = cfg->bond_u0[b];
+    for (const Leg& l : legs) {
+      int q = fill[l.i]++;
+      part[q] = l.j;
+      r0v[q] = l.r0;
+      u0v[q] = l.u0;
+      q = fill[l.j]++;
+      part[q] = l.i;
+      r0v[q] = l.r0;
+      u0v[q] = l.u0;
     }
     A_(c->boff, N + 1);
     A_(c->bpart, part.size());
@@ -2405,6 +2554,42 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     HIP_TRY(hipMemcpy(c->br0, r0v.data(), sizeof(double) * r0v.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->bu0, u0v.data(), sizeof(double) * u0v.size(), hipMemcpyHostToDevice));
     c->any_bonds = true;
+  }
+  // angles (pyrdsim.pyx:1322-1407): table in definition order and a CSR by
+  // particle of (angle, role) memberships
+  if (cfg->n_angles > 0) {
+    const int64_t na = cfg->n_angles;
+    std::vector<int4> ap(na);
+    std::vector<double2> prm(na);
+    std::vector<int> off(N + 1, 0);
+    for (int64_t q = 0; q < na; ++q) {
+      ap[q] = make_int4(cfg->angle_i[q], cfg->angle_j[q], cfg->angle_k[q], 0);
+      prm[q] = make_double2(cfg->angle_theta0[q], cfg->angle_k0[q]);
+      off[ap[q].x + 1]++;
+      off[ap[q].y + 1]++;
+      off[ap[q].z + 1]++;
+    }
+    for (size_t i = 0; i < N; ++i) off[i + 1] += off[i];
+    std::vector<int2> ent(off[N]);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int64_t q = 0; q < na; ++q) {
+      ent[fill[ap[q].x]++] = make_int2((int)q, 0);
+      ent[fill[ap[q].y]++] = make_int2((int)q, 1);
+      ent[fill[ap[q].z]++] = make_int2((int)q, 2);
+    }
+    A_(c->aoff, N + 1);
+    A_(c->aent, ent.size());
+    A_(c->apart, na);
+    A_(c->aprm, na);
+    HIP_TRY(hipMemcpy(c->aoff, off.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->aent, ent.data(), sizeof(int2) * ent.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->apart, ap.data(), sizeof(int4) * na, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->aprm, prm.data(), sizeof(double2) * na, hipMemcpyHostToDevice));
+    // the closures read the flag set for the last angle defined
+    // (pyrdsim.pyx:1334-1338, 1378); np.isclose(theta0, pi, atol=1e-5)
+    const double last = cfg->angle_theta0[na - 1];
+    c->angle_pi = std::fabs(last - M_PI) <= 1e-5 + 1e-5 * M_PI;
+    c->angle_thr = cfg->angle_threshold;
   }
 #undef A_
   *out = c;
@@ -2514,6 +2699,8 @@ int rds_sync(rds_ctx* c, rds_status* out) {
   s.rebuilds = (int64_t)st.rebuilds;
   s.list_overflow = (int64_t)st.overflow;
   s.list_entries = (int64_t)st.entries;
+  s.angle_error = st.angle;
+  if (st.angle && !st.bounds) s.error_value = st.angle_val;
   if (st.bounds) {
     s.error_step = (int64_t)(st.err_key >> 32);
     unsigned lo = (unsigned)(st.err_key & 0xffffffffu);
@@ -2529,6 +2716,9 @@ int rds_sync(rds_ctx* c, rds_status* out) {
                 "x[%d,%d] = %.17g out of bounds. Maybe the timestep is too large for the given "
                 "the interaction parameters? (step %lld)",
                 s.error_particle, s.error_axis, s.error_value, (long long)s.error_step);
+  if (st.angle)
+    return fail(RDS_E_ANGLE, "Inner product of units vectors is smaller than -1. Value = %.17g",
+                st.angle_val);
   if (st.stream) return fail(RDS_E_STREAM, "injected random stream exhausted");
   return RDS_OK;
 }
@@ -2538,9 +2728,7 @@ int rds_compute_forces(rds_ctx* c, double* forces) {
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   Buf& cur = c->buf[c->parity];
   if (c->any_bonds)
-    hipLaunchKernelGGL(k_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
-                       (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, c->slot_of,
-                       c->boff, c->bpart, c->br0, c->bu0, c->fbond, c->cpos);
+    launch_bonds(c, cur);
   StepArgs a = make_args(c, cur);
   a.fout = (double2*)forces;
   launch_force(c, a, MODE_FORCE);

@@ -131,12 +131,16 @@ class Engine:
     keeps Verlet lists within (pair range + skin), rebuilt with the sort only
     after some particle moved more than skin/2 (list_capacity entries per
     particle, 0 = 64; a particle with more falls back to the stencil scan).
+
+    bonds = (i, j, r0, u0) and angles = (i1, i2, i3, theta0 [rad], k0 [per
+    rad^2]) are arrays over global particle indices, in definition order
+    (pyrdsim.pyx:1287-1407); i2 is an angle's vertex.
     """
 
     def __init__(self, models, replicas, precision='mixed', rng='philox',
                  reaction_prob='linear', bonds=None, has_molecules=False,
                  tile_particles=0, graph_steps=0, tile_rows=1, device='cuda',
-                 skin=0.0, list_capacity=0):
+                 skin=0.0, list_capacity=0, angles=None, angle_threshold=0.1):
         self.lib = L.load()
         if not torch.cuda.is_available():
             raise RuntimeError('the engine needs a ROCm GPU; none is visible')
@@ -176,6 +180,21 @@ class Engine:
             cfg.bond_j = bj.ctypes.data_as(C.POINTER(C.c_int32))
             cfg.bond_r0 = br.ctypes.data_as(C.POINTER(C.c_double))
             cfg.bond_u0 = bu.ctypes.data_as(C.POINTER(C.c_double))
+        self._angle_arrays = None
+        cfg.angle_threshold = float(angle_threshold)
+        if angles is not None and len(angles[0]) > 0:
+            ai, aj, ak = (np.ascontiguousarray(a, dtype=np.int32) for a in angles[:3])
+            at = np.ascontiguousarray(angles[3], dtype=np.float64)
+            ak0 = np.ascontiguousarray(angles[4], dtype=np.float64)
+            if not (len(ai) == len(aj) == len(ak) == len(at) == len(ak0)):
+                raise ValueError('angle arrays differ in length')
+            self._angle_arrays = (ai, aj, ak, at, ak0)
+            cfg.n_angles = len(ai)
+            cfg.angle_i = ai.ctypes.data_as(C.POINTER(C.c_int32))
+            cfg.angle_j = aj.ctypes.data_as(C.POINTER(C.c_int32))
+            cfg.angle_k = ak.ctypes.data_as(C.POINTER(C.c_int32))
+            cfg.angle_theta0 = at.ctypes.data_as(C.POINTER(C.c_double))
+            cfg.angle_k0 = ak0.ctypes.data_as(C.POINTER(C.c_double))
         self.precision = precision
         self.rng = rng
         ctx = C.c_void_p()

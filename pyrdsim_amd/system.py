@@ -13,7 +13,6 @@ types as pyrdsim/pyrdsim.pyx.  Differences, all deliberate:
   ``neighbor_list_update_frequency`` steps, type-gated at refresh time;
   ``rs``, ``cell_length_ratio`` and ``neighbor_list_update_frequency`` are
   accepted and kept but do not change results;
-* angle potentials are not implemented (NotImplementedError when present);
 * the trajectory file is HDF5 through h5py when it is importable, else an
   ``.npz`` file with the same datasets.
 
@@ -278,22 +277,23 @@ class system:
             f.write(_parse.format_state(self.molecule_ids, self.particle_types, self.x))
 
     def _bonds(self):
-        """Bond list from the molecule definitions (pyrdsim.pyx:1287-1320)."""
-        bi, bj, br, bu = [], [], [], []
+        """Bond and angle lists from the molecule definitions, in the
+        reference's definition order (pyrdsim.pyx:1287-1407): bonds
+        (i, j, r0, u0) and angles (i1, i2, i3, theta0, k0)."""
+        bonds = ([], [], [], [])
+        angles = ([], [], [], [], [])
         for i in range(self.N_molecules):
             j = int(self.first_index_of_molecule[i])
             mol_id = int(self.molecule_ids[j, 2])
             for it in self.molecule_interactions.get(mol_id, []):
+                p = [q + j for q in it['participants']]
                 if it['type'] == 'bond':
-                    bi.append(it['participants'][0] + j)
-                    bj.append(it['participants'][1] + j)
-                    br.append(it['r0'])
-                    bu.append(it['u0'])
+                    for col, v in zip(bonds, (p[0], p[1], it['r0'], it['u0'])):
+                        col.append(v)
                 elif it['type'] == 'angle':
-                    raise NotImplementedError(
-                        'angle potentials (pyrdsim.pyx:1322-1407) are not implemented by '
-                        'the MI355X engine')
-        return bi, bj, br, bu
+                    for col, v in zip(angles, (p[0], p[1], p[2], it['theta0'], it['u0'])):
+                        col.append(v)
+        return bonds, angles
 
     def _trajectory_io(self):
         from . import trajectory
@@ -306,12 +306,14 @@ class system:
         model, rates = build_model(self.interactions, self.reactions, self.N_species)
         rep = make_replica(0, self.N_particles, self.L, self.periodic, self.dt,
                            self.tau_by_tauD, rates, self.seed)
-        bonds = self._bonds()
+        bonds, angles = self._bonds()
         has_mol = self.N_molecules < self.N_particles
         eng = Engine([model], [rep], precision=self.precision, rng='philox',
                      reaction_prob=self.reaction_probability,
                      bonds=bonds if len(bonds[0]) else None, has_molecules=has_mol,
-                     graph_steps=int(self.graph_steps), skin=float(self.skin))
+                     graph_steps=int(self.graph_steps), skin=float(self.skin),
+                     angles=angles if len(angles[0]) else None,
+                     angle_threshold=self.angle_pi_approximation_threshold)
         eng.set_random_force(random_force)
         return eng
 

@@ -38,7 +38,7 @@ def engine_for(s, precision='fp64', rng='philox', seed=2001, graph_steps=0,
     eng = Engine([model], [rep], precision=precision, rng=rng, reaction_prob=reaction_prob,
                  bonds=s['bonds'], has_molecules=has_mol, graph_steps=graph_steps,
                  tile_particles=tile_particles, tile_rows=tile_rows, skin=skin,
-                 list_capacity=list_capacity)
+                 list_capacity=list_capacity, angles=s.get('angles'))
     eng.set_state(s['x'], s['types'], s['molecule_ids'][:, 0] if has_mol else None)
     return eng
 
@@ -50,9 +50,12 @@ def oracle_for(s, rng='philox', seed=2001, nl='exact', **kw):
     if s['bonds'] is not None:
         bi, bj, br, bu = s['bonds']
         bonds = list(zip(bi, bj, br, bu))
+    angles = ()
+    if s.get('angles') is not None:
+        angles = list(zip(*s['angles']))
     o = Oracle(s['x'], s['types'], s['L'], s['periodic'], s['dt'], m['tau_by_tauD'],
                m['interactions'], m['reactions'], molecules=s['molecule_ids'][:, 0],
-               bonds=bonds, seed=seed, rng=rng, nl=nl, **kw)
+               bonds=bonds, seed=seed, rng=rng, nl=nl, angles=angles, **kw)
     return o
 
 
@@ -110,6 +113,11 @@ def force_scale(s, types=None):
         bi, bj, br, bu = s['bonds']
         np.add.at(scale, bi, bu * br)
         np.add.at(scale, bj, bu * br)
+    if s.get('angles') is not None:
+        # an angle's force at one radian from rest over a leg of length 1
+        i1, i2, i3, _, k0 = s['angles']
+        for p in (i1, i2, i3):
+            np.add.at(scale, p, np.asarray(k0))
     return scale
 
 

@@ -106,3 +106,48 @@ def test_config_errors_before_any_device_call():
         assert lib.rds_ctx_create(C.byref(cfg), C.byref(ctx)) == _lib.E_ARG
         assert text in lib.rds_last_error(), lib.rds_last_error()
         assert not ctx.value
+
+
+def test_topology_errors_before_any_device_call():
+    """Bonds and angles that name a particle twice, a missing particle or two
+    replicas are refused before touching a GPU."""
+    import numpy as np
+    lib = _lib.load()
+    ctx = C.c_void_p()
+    i32 = lambda v: np.ascontiguousarray(v, dtype=np.int32)
+    f64 = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+    P32, PD = C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    cases = (((3, 4, 3), b'invalid particles'), ((3, 4, 16), b'invalid particles'),
+             ((3, -1, 5), b'invalid particles'))
+    for (i, j, k), text in cases:
+        cfg, keep = _config(_lib.FP64, 0.0)
+        arrs = (i32([i]), i32([j]), i32([k]), f64([2.0]), f64([10.0]))
+        cfg.n_angles = 1
+        cfg.angle_i, cfg.angle_j, cfg.angle_k = (a.ctypes.data_as(P32) for a in arrs[:3])
+        cfg.angle_theta0, cfg.angle_k0 = (a.ctypes.data_as(PD) for a in arrs[3:])
+        assert lib.rds_ctx_create(C.byref(cfg), C.byref(ctx)) == _lib.E_ARG
+        assert text in lib.rds_last_error(), lib.rds_last_error()
+        assert not ctx.value
+    cfg, keep = _config(_lib.FP64, 0.0)
+    cfg.n_angles = 2
+    assert lib.rds_ctx_create(C.byref(cfg), C.byref(ctx)) == _lib.E_ARG
+    assert b'angle arrays missing' in lib.rds_last_error()
+    # two replicas of 16 particles: a bond (15, 16) spans them
+    model = _lib.Model()
+    model.n_types = 1
+    reps = (_lib.Replica * 2)()
+    for r in reps:
+        r.n_particles = 16
+        r.L[0] = r.L[1] = 10.0
+        r.periodic[0] = r.periodic[1] = 1
+        r.dt = 1e-4
+    cfg = _lib.Config()
+    cfg.n_models, cfg.n_replicas = 1, 2
+    cfg.models = C.pointer(model)
+    cfg.replicas = C.cast(reps, C.POINTER(_lib.Replica))
+    bi, bj, br, bu = i32([15]), i32([16]), f64([1.0]), f64([1.0])
+    cfg.n_bonds = 1
+    cfg.bond_i, cfg.bond_j = bi.ctypes.data_as(P32), bj.ctypes.data_as(P32)
+    cfg.bond_r0, cfg.bond_u0 = br.ctypes.data_as(PD), bu.ctypes.data_as(PD)
+    assert lib.rds_ctx_create(C.byref(cfg), C.byref(ctx)) == _lib.E_ARG
+    assert b'spans replicas' in lib.rds_last_error(), lib.rds_last_error()

@@ -148,3 +148,28 @@ def test_system_defaults_and_errors(tmp_path):
         system(dict(base, periodic=np.array([1.0])))
     with pytest.raises(RuntimeError, match='not recognized'):
         system(dict(base, interactions=[dict(type='morse', pair=[0, 0], u0=1, rc=1)]))
+
+
+def test_bond_and_angle_lists_of_molecules(tmp_path):
+    """The facade's bond and angle arrays: participants offset by each
+    molecule's first index, in the reference's definition order
+    (pyrdsim.pyx:1287-1407); angles in rad and per rad^2."""
+    ic = tmp_path / 'ic.txt'
+    x = np.arange(16, dtype=float).reshape(8, 2) + 1.0
+    mol = np.array([[m, k, 0] for m in range(2) for k in range(4)], dtype=np.intc)
+    _parse.write_initial_conditions(str(ic), x, np.array([0, 1, 1, 1] * 2), mol)
+    base = dict(parameters_filename=str(tmp_path / 'none.txt'), n_dim=2,
+                l=np.array([20., 20.]), dt=1e-5, n_steps=10, n_out=10, interactions=[],
+                reactions=[], tau_by_tauD=[1.0, 1.0], initial_conditions_filename=str(ic),
+                bonded_interactions_filename=os.path.join(EX, 'patchy',
+                                                          'bonded_interactions_3.txt'))
+    s = system(dict(base, seed=1))
+    s.load_initial_conditions_from_file()
+    bonds, angles = s._bonds()
+    assert bonds[0] == [0, 0, 0, 4, 4, 4] and bonds[1] == [1, 2, 3, 5, 6, 7]
+    assert bonds[2] == [1.0] * 6 and bonds[3] == [400.0] * 6
+    assert angles[0] == [1, 2, 3, 5, 6, 7]
+    assert angles[1] == [0, 0, 0, 4, 4, 4]
+    assert angles[2] == [2, 3, 1, 6, 7, 5]
+    assert np.allclose(angles[3], 2 * np.pi / 3, rtol=0, atol=1e-15)
+    assert np.allclose(angles[4], 0.171934 * (180 / np.pi) ** 2, rtol=1e-15)
