@@ -276,6 +276,85 @@ class system:
         with open(filename, 'w') as f:
             f.write(_parse.format_state(self.molecule_ids, self.particle_types, self.x))
 
+    def create_initial_conditions(self, N_particles, particle_types,
+                                  reference_configurations=None,
+                                  min_distance_threshold=None,
+                                  min_distance_threshold_sim=None,
+                                  seed=None, relax_steps=None, relax_random_force=False):
+        """pyrdsim.pyx:2634-2828, seeded.  N_particles = {molecule_id: count},
+        particle_types = {molecule_id: [type of each atom]},
+        reference_configurations = {molecule_id: filename} (optional; else the
+        geometry minimising the molecule's bonded energy).  Molecules are
+        placed rigidly at random with every atom at least
+        min_distance_threshold (default rs/4) from the others, then relaxed
+        once on the GPU for relax_steps (default N_steps) steps without noise
+        (relax_random_force=False); a relaxed state whose smallest distance
+        between molecules is below min_distance_threshold_sim (default rs/10)
+        is placed again.  Returns the reference's output array (molecule
+        number, index in molecule, molecule id, type, positions) and writes
+        initial_conditions_filename.  seed=None uses the system's seed."""
+        from . import initial_conditions as icg
+        if self.N_dim != 2:
+            raise NotImplementedError('the MI355X engine integrates two-dimensional systems')
+        rng = np.random.Generator(np.random.PCG64(self.seed if seed is None else seed))
+        if min_distance_threshold is None:
+            min_distance_threshold = self.rs / 4.
+        if min_distance_threshold_sim is None:
+            min_distance_threshold_sim = self.rs / 10.
+        n_atoms = {i: self.molecule_parameters[i]['n_atoms'] for i in self.molecule_interactions}
+        refs = {}
+        if not reference_configurations:
+            for i in self.molecule_interactions:
+                refs[i] = icg.reference_molecule(self.molecule_interactions[i], n_atoms[i],
+                                                 self.N_dim, rng)
+        else:
+            for i, filename in reference_configurations.items():
+                refs[i] = np.atleast_2d(np.genfromtxt(filename))
+                if len(refs[i]) != n_atoms[i]:
+                    raise RuntimeError(("Reference configuration for molecule {0}"
+                                        " has {1} atoms, but interaction parameters for that"
+                                        " molecule demand {2} atoms").format(
+                                            i, len(refs[i]), n_atoms[i]))
+        L = np.asarray(self.L, dtype=float)
+        steps = self.N_steps if relax_steps is None else int(relax_steps)
+        for _ in range(10):
+            out = icg.place_molecules(N_particles, particle_types, refs, L, self.periodic,
+                                      min_distance_threshold, rng)
+            self._set_molecules(out)
+            if steps > 0:
+                eng = self._engine(relax_random_force)
+                try:
+                    eng.set_state(self.x, self.particle_types,
+                                  self.molecule_ids[:, 0] if eng.has_molecules else None)
+                    eng.run(steps)
+                    x, _ = eng.state()
+                except RuntimeError:
+                    continue
+                finally:
+                    eng.close()
+                if np.isnan(x).any():
+                    continue
+                d = icg.min_intermolecular_distance(x, self.molecule_ids[:, 0], L,
+                                                    self.periodic, min_distance_threshold_sim)
+                if d < min_distance_threshold_sim:
+                    continue
+                out[:, 4:] = x
+                self.x = x
+            self.write_current_state_to_text_file(filename=self.initial_conditions_filename)
+            return out
+        raise RuntimeError('Could not create an initial condition: relaxed states kept '
+                           'molecules closer than {0}'.format(min_distance_threshold_sim))
+
+    def _set_molecules(self, out):
+        """State arrays of an output array of create_initial_conditions."""
+        self.molecule_ids = np.array(out[:, :3], dtype=np.intc)
+        self.particle_types = np.array(out[:, 3], dtype=np.intc)
+        self.x = np.array(out[:, 4:], dtype=np.float64)
+        self.N_particles = len(out)
+        first = np.flatnonzero(self.molecule_ids[:, 1] == 0)
+        self.N_molecules = len(first)
+        self.first_index_of_molecule = first.astype(np.intc)
+
     def _bonds(self):
         """Bond and angle lists from the molecule definitions, in the
         reference's definition order (pyrdsim.pyx:1287-1407): bonds

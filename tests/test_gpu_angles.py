@@ -215,3 +215,43 @@ def test_facade_reads_angles_from_files(gpu, tmp_path):
     d -= s['L'] * np.round(d / s['L'])
     assert np.abs(d).max() < 1e-9
     assert np.array_equal(out['particle_types'], t)
+
+
+def test_reference_four_patch_file_forces(gpu, tmp_path):
+    """The reference's bonded_interactions/4.txt mixes 90 and 180 degree
+    angles with a 180 degree one last, so every angle takes the straight-angle
+    flag.  Molecules placed from that file's minimum-energy cross, jittered:
+    bond and angle lists built by the facade, forces against the oracle."""
+    from pyrdsim_amd import initial_conditions as icg
+    from pyrdsim_amd.system import system
+    ex = os.path.join(os.path.dirname(__file__), 'golden', 'examples', 'patchy')
+    base = dict(parameters_filename=str(tmp_path / 'none.txt'), n_dim=2,
+                l=np.array([70., 70.]), dt=1e-5, n_steps=10, n_out=10,
+                interactions=[dict(type='lj', pair=[0, 0], u0=1.0, r0=2.5, n=10, rc=5.0),
+                              dict(type='tanh', pair=[2, 2], u0=20.0, r0=0.2, a=0.1, rc=1.0)],
+                reactions=[], tau_by_tauD=[1.0, 4.5, 4.5],
+                bonded_interactions_filename=os.path.join(ex, 'bonded_interactions_4.txt'))
+    sim = system(dict(base, seed=3))
+    rng = np.random.Generator(np.random.PCG64(61))
+    ref = icg.reference_molecule(sim.molecule_interactions[0], 5, 2, rng)
+    out = icg.place_molecules({0: 100}, {0: [0, 1, 2, 1, 2]}, {0: ref}, np.array([70., 70.]),
+                              [1, 1], 2.6, rng)
+    sim._set_molecules(out)
+    x = sim.x + rng.uniform(-0.08, 0.08, sim.x.shape)
+    bonds, angles = sim._bonds()
+    assert len(angles[0]) == 6 * 100 and np.isclose(angles[3][-1], np.pi)
+    s = dict(name='patchy4', x=x % 70.0, types=sim.particle_types,
+             molecule_ids=sim.molecule_ids, L=np.array([70., 70.]),
+             periodic=np.array([1.0, 1.0]), dt=1e-5,
+             model=dict(interactions=base['interactions'], reactions=[],
+                        tau_by_tauD=[1.0, 4.5, 4.5]),
+             bonds=tuple(np.asarray(v) for v in bonds),
+             angles=tuple(np.asarray(v) for v in angles))
+    o = oracle_for(s)
+    o.init()
+    _, _, a = o.state()
+    scale = force_scale(s) + np.abs(a).max(axis=1)
+    for precision, rtol in (('fp64', 1e-10), ('mixed', 1e-5)):
+        f = engine_for(s, precision=precision).forces()
+        err = np.abs(f - a).max(axis=1)
+        assert not ((err > rtol * scale) & ~near_cutoff(s)).any(), (precision, err.max())
