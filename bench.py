@@ -11,7 +11,9 @@ together with FP32 pair forces and FP64 positions, hipGraph-captured.  Beside it
   HIP events on the engine's stream during an eager pass over the same state,
   against its algorithmic HBM bytes (48 B per particle + 4 B per cell);
 * `cpu_baseline`: the CPU oracle in reference-faithful mode (Verlet list on the
-  reference's grid, FP64, one core) on a bounded sample of the same replicas.
+  reference's grid, FP64, one core) on a bounded sample of the same replicas;
+* `analysis`: the RDF pair histogram, type counts and cluster labels of the
+  whole ensemble on the device, beside the host RDF of one replica.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -181,6 +183,40 @@ def accuracy_leg(plan, states, models, reps, x0, t0s, graph_steps, tiling, steps
     return out
 
 
+def analysis_leg(eng, plan, reps):
+    """RDF pair histogram (r <= 5, bins of 0.02), type counts and cluster
+    labels (active particles, links <= 1.5) of the whole ensemble on the
+    device, wall time including the copy of the results to the host; the
+    same RDF on the host (cKDTree) for the median-size replica, per particle."""
+    from pyrdsim_amd import analysis
+    eng.pair_histogram(5.0, 0.02)
+    t = time.perf_counter()
+    hist = eng.pair_histogram(5.0, 0.02)
+    t_rdf = time.perf_counter() - t
+    t = time.perf_counter()
+    eng.type_counts()
+    t_cnt = time.perf_counter() - t
+    t = time.perf_counter()
+    eng.cluster_labels(1.5, types=(1,))
+    t_cl = time.perf_counter() - t
+    x, _ = eng.state()
+    n = np.array([p['n'] for p in plan])
+    off = np.concatenate([[0], np.cumsum(n)])
+    r = int(np.argsort(n, kind='stable')[len(n) // 2])
+    rep = reps[r]
+    L = np.array(rep.L[:])
+    per = [1 if rep.periodic[0] else -1, 1 if rep.periodic[1] else -1]
+    t = time.perf_counter()
+    analysis.rdf(x[off[r]:off[r + 1]], L, per, 5.0, 0.02)
+    t_host = time.perf_counter() - t
+    n_tot = int(n.sum())
+    return dict(particles=n_tot, pairs=int(hist.sum()), rdf_device_ms=round(1e3 * t_rdf, 3),
+                type_counts_device_ms=round(1e3 * t_cnt, 3),
+                clusters_device_ms=round(1e3 * t_cl, 3),
+                rdf_host_ms_per_1e6_particles=round(1e3 * t_host / n[r] * 1e6, 1),
+                host_sample='analysis.rdf on replica %d (N=%d), one core' % (r, int(n[r])))
+
+
 def measure_traffic(cache, rows, per, replicas, timeout=300):
     """HBM bytes per force-kernel launch from two rocprofv3 counter passes
     (FETCH_SIZE, then WRITE_SIZE) over a few eager steps of this workload in a
@@ -286,6 +322,7 @@ def main():
                     help='run one elementwise kernel of known traffic (256 MiB read + 256 MiB '
                          'written) first, to scale HBM counters in a profile of this command')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-analysis', action='store_true')
     ap.add_argument('--no-traffic', dest='traffic', action='store_false',
                     help='skip the rocprofv3 counter passes for roofline.traffic')
     args = ap.parse_args()
@@ -373,6 +410,10 @@ def main():
                     algorithmic_bytes=int(force_bytes),
                     avg_ms={k: round(v[0], 5) for k, v in kt.items() if v[1]})
     tiling = eng.tiling()
+    ana = None
+    if not args.no_analysis:
+        ana = analysis_leg(eng, plan, reps)
+        print('# analysis: %s' % ana, file=sys.stderr, flush=True)
     del eng
     torch.cuda.synchronize()
 
@@ -465,6 +506,7 @@ def main():
         'accuracy': accuracy,
         'compensated_positions': compensated,
         'batched_fp64': batched_fp64,
+        'analysis': ana,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))

@@ -236,6 +236,30 @@ int rds_grid(rds_ctx* ctx, int32_t* cells_xy, double* cell_len, int32_t* cell_ba
  * order (the neighbour sets of create_neighbor_list, pyrdsim.pyx:1448-1587). */
 int rds_neighbor_counts(rds_ctx* ctx, double r, int32_t* counts);
 
+/* Analysis of the current state on the device.  The reference computes none
+ * of these: it writes trajectory.h5, and its examples post-process that file
+ * on the host (create_video.py).  These replace that host work; r_max and
+ * r_link are at most half of every periodic box length.
+ *
+ * rds_pair_histogram: per replica, the unordered pairs of particles whose
+ * types are in type_mask (bit t = type t; 0 = every type) at minimum-image
+ * distance r <= r_max, counted in bin floor(r / dr) when that is below n_bins
+ * (1..4096).  hist: device [n_replicas][n_bins] uint64, overwritten.  The
+ * RDF is these counts over the ideal-gas shell counts. */
+int rds_pair_histogram(rds_ctx* ctx, double r_max, double dr, int32_t n_bins,
+                       uint32_t type_mask, uint64_t* hist);
+
+/* Particles of each type per replica: device [n_replicas][RDS_MAX_TYPES]
+ * int64, overwritten (activated fraction = count of the active type / n). */
+int rds_type_counts(rds_ctx* ctx, int64_t* counts);
+
+/* Clusters: connected components of the particles whose types are in
+ * type_mask (0 = every type), two particles linked when r <= r_link
+ * (minimum image).  labels: device [N] int32 in caller order, the smallest
+ * particle index of the particle's cluster, or -1 for a particle of another
+ * type.  Waits for the stream (one host check per propagation pass). */
+int rds_cluster_labels(rds_ctx* ctx, double r_link, uint32_t type_mask, int32_t* labels);
+
 /* Times n_steps eager steps with HIP events around every launch on the
  * context's stream; avg_ms[k] for k = 0 force+integrate, 1 scan, 2 scatter,
  * 3 reaction, 4 bonds, 5 list mode's binning + list build (the conditional

@@ -110,6 +110,9 @@ SIGNATURES = {
     'rds_cell_ids': (C.c_int, [_VP, _VP]),
     'rds_grid': (C.c_int, [_VP, _VP, _VP, _VP]),
     'rds_neighbor_counts': (C.c_int, [_VP, C.c_double, _VP]),
+    'rds_pair_histogram': (C.c_int, [_VP, C.c_double, C.c_double, C.c_int32, C.c_uint32, _VP]),
+    'rds_type_counts': (C.c_int, [_VP, _VP]),
+    'rds_cluster_labels': (C.c_int, [_VP, C.c_double, C.c_uint32, _VP]),
     'rds_time_kernels': (C.c_int, [_VP, C.c_int64, _VP, _VP]),
     'rds_debug_fill_lds': (C.c_int, [_VP, C.c_uint32]),
     'rds_set_tiling': (C.c_int, [_VP, C.c_int32, C.c_int32]),

@@ -53,6 +53,27 @@ def rdf(x, L, periodic, r_max=5.0, dr=0.02, select=None):
     return 0.5 * (edges[1:] + edges[:-1]), g
 
 
+def rdf_from_counts(counts, n, L, r_max=5.0, dr=0.02):
+    """g(r) at bin centres from a pair histogram (the engine's
+    pair_histogram), with the normalisation of rdf()."""
+    edges = np.arange(0.0, r_max + dr / 2, dr)
+    counts = np.asarray(counts, dtype=np.float64)[:len(edges) - 1]
+    if n < 2:
+        return 0.5 * (edges[1:] + edges[:-1]), np.zeros(len(edges) - 1)
+    rho = n / (float(L[0]) * float(L[1]))
+    shell = np.pi * (edges[1:] ** 2 - edges[:-1] ** 2)
+    return 0.5 * (edges[1:] + edges[:-1]), 2.0 * counts / (n * rho * shell)
+
+
+def cluster_sizes_from_labels(labels):
+    """Cluster sizes from per-particle cluster labels (-1 = not in a cluster)."""
+    lab = np.asarray(labels)
+    lab = lab[lab >= 0]
+    if lab.size == 0:
+        return np.zeros(0, dtype=int)
+    return np.unique(lab, return_counts=True)[1]
+
+
 def rdf_error(g, g_ref):
     return float(np.abs(g - g_ref).sum() / np.abs(g_ref).sum())
 

@@ -125,6 +125,7 @@ struct DStatus {
   unsigned long long rebuilds, overflow, entries;
   unsigned done;               // blocks of the list build that finished (reset by the last)
   int angle;                   // an angle's cosine fell below -1.0000001
+  int cl_changed;              // cluster labels: a label changed in this pass
   double angle_val;            // that cosine
 };
 
@@ -1651,6 +1652,149 @@ __global__ void k_gather_out(int n_total, const double2* __restrict__ pos,
   if (to) to[p] = type[s];
 }
 
+// ---------------------------------------------------------------------------
+// analysis on the sorted state: pair-distance histogram (RDF), type counts,
+// cluster labels.  One block per replica slot block (sblocks), one thread per
+// slot; cells of the replica within a radius, every cell once.
+// ---------------------------------------------------------------------------
+
+// Calls f(j) for every slot j in the cells within `rad` of xi's cell (rings of
+// ceil(rad / cell length) cells; an axis the rings cover is walked once).
+template <typename F>
+__device__ __forceinline__ void for_cells_within(const DRep& R, const int* __restrict__ cstart,
+                                                 double2 xi, double rad, F f) {
+  const int cx = cell_coord(xi.x, R.cl[0], R.lx), cy = cell_coord(xi.y, R.cl[1], R.ly);
+  const int kx = (int)ceil(rad / R.cl[0]), ky = (int)ceil(rad / R.cl[1]);
+  const bool fx = 2 * kx + 1 >= R.lx, fy = 2 * ky + 1 >= R.ly;
+  const int x0 = fx ? 0 : cx - kx, nx = fx ? R.lx : 2 * kx + 1;
+  const int y0 = fy ? 0 : cy - ky, ny = fy ? R.ly : 2 * ky + 1;
+  for (int oy = 0; oy < ny; ++oy) {
+    int y = y0 + oy;
+    if (y < 0 || y >= R.ly) {
+      if (!R.py) continue;
+      y = y < 0 ? y + R.ly : y - R.ly;
+    }
+    for (int ox = 0; ox < nx; ++ox) {
+      int x = x0 + ox;
+      if (x < 0 || x >= R.lx) {
+        if (!R.px) continue;
+        x = x < 0 ? x + R.lx : x - R.lx;
+      }
+      const int c = R.cbase + y * R.lx + x;
+      const int j1 = cstart[c + 1];
+      for (int j = cstart[c]; j < j1; ++j) f(j);
+    }
+  }
+}
+
+__device__ __forceinline__ bool type_selected(int t, unsigned mask) {
+  return t >= 0 && t < 32 && ((mask >> t) & 1u);
+}
+
+// squared minimum-image distance, each product and the sum rounded (no fused
+// multiply-add), as a host computes dx*dx + dy*dy
+__device__ __forceinline__ double dist2(const DRep& R, double2 a, double2 b) {
+  double dx = b.x - a.x, dy = b.y - a.y;
+  if (R.px) dx = min_image(dx, R.L[0]);
+  if (R.py) dy = min_image(dy, R.L[1]);
+  return __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+}
+
+// unordered pairs (from the smaller slot) of selected types with r <= r_max
+// into bins floor(r / dr) < nbins, per replica
+__global__ void k_pair_hist(const DRep* __restrict__ rep, const int4* __restrict__ sblocks,
+                            const int* __restrict__ cstart, const double2* __restrict__ pos,
+                            const int* __restrict__ type, int cpos, double r2max, double dr,
+                            double rad, int nbins, unsigned mask,
+                            unsigned long long* __restrict__ hist) {
+  extern __shared__ unsigned s_hist[];
+  const int4 sb = sblocks[blockIdx.x];
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0u;
+  __syncthreads();
+  const int s = sb.y + (int)threadIdx.x;
+  if (s < sb.z && type_selected(type[s], mask)) {
+    const DRep& R = rep[sb.x];
+    const double2 xi = rd_pos(pos, s, cpos);
+    for_cells_within(R, cstart, xi, rad, [&](int j) {
+      if (j <= s || !type_selected(type[j], mask)) return;
+      const double d2 = dist2(R, xi, rd_pos(pos, j, cpos));
+      if (d2 > r2max) return;
+      const int b = (int)floor(sqrt(d2) / dr);
+      if (b < nbins) atomicAdd(&s_hist[b], 1u);
+    });
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+    if (s_hist[b]) atomicAdd(&hist[(size_t)sb.x * nbins + b], (unsigned long long)s_hist[b]);
+}
+
+__global__ void k_type_counts(const int4* __restrict__ sblocks, const int* __restrict__ type,
+                              unsigned long long* __restrict__ counts) {
+  __shared__ unsigned s_cnt[MAXT];
+  const int4 sb = sblocks[blockIdx.x];
+  if (threadIdx.x < MAXT) s_cnt[threadIdx.x] = 0u;
+  __syncthreads();
+  const int s = sb.y + (int)threadIdx.x;
+  if (s < sb.z) {
+    const int t = type[s];
+    if (t >= 0 && t < MAXT) atomicAdd(&s_cnt[t], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < MAXT && s_cnt[threadIdx.x])
+    atomicAdd(&counts[(size_t)sb.x * MAXT + threadIdx.x], (unsigned long long)s_cnt[threadIdx.x]);
+}
+
+// Cluster labels by min-label propagation with hooking and pointer jumping,
+// iterated from the host until no label changes.  A label is the index (pid)
+// of a particle of the same cluster and only decreases, so the fixed point
+// is the smallest index of each cluster.
+__global__ void k_cl_init(int n_total, const int* __restrict__ type, const int* __restrict__ pid,
+                          unsigned mask, int* __restrict__ label, int* __restrict__ slot_of) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  label[s] = type_selected(type[s], mask) ? pid[s] : -1;
+  slot_of[pid[s]] = s;
+}
+
+__global__ void k_cl_prop(const DRep* __restrict__ rep, const int4* __restrict__ sblocks,
+                          const int* __restrict__ cstart, const double2* __restrict__ pos,
+                          int cpos, double rl2, double rad, int* label,
+                          const int* __restrict__ slot_of, DStatus* status) {
+  const int4 sb = sblocks[blockIdx.x];
+  const int s = sb.y + (int)threadIdx.x;
+  if (s >= sb.z) return;
+  const int l = label[s];
+  if (l < 0) return;
+  const DRep& R = rep[sb.x];
+  const double2 xi = rd_pos(pos, s, cpos);
+  int m = l;
+  for_cells_within(R, cstart, xi, rad, [&](int j) {
+    const int lj = label[j];
+    if (lj < 0 || lj >= m || j == s) return;
+    if (dist2(R, xi, rd_pos(pos, j, cpos)) <= rl2) m = lj;
+  });
+  if (m < l) {
+    atomicMin(&label[s], m);
+    atomicMin(&label[slot_of[l]], m);  // the particle this one pointed to joins too
+    status->cl_changed = 1;
+  }
+}
+
+__global__ void k_cl_jump(int n_total, int* label, const int* __restrict__ slot_of) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  const int l = label[s];
+  if (l < 0) return;
+  const int l2 = label[slot_of[l]];
+  if (l2 >= 0 && l2 < l) atomicMin(&label[s], l2);
+}
+
+__global__ void k_cl_out(int n_total, const int* __restrict__ label, const int* __restrict__ pid,
+                         int* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n_total) out[pid[s]] = label[s];
+}
+
 __global__ void k_cells_out(int n_total, const DRep* __restrict__ rep,
                             const int* __restrict__ rep_of, const double2* __restrict__ pos,
                             const int* __restrict__ pid, int* __restrict__ out, int cpos) {
@@ -1735,6 +1879,7 @@ struct rds_ctx {
   double* bu0 = nullptr;
   double2* fbond = nullptr;
   int* slot_of = nullptr;
+  int* clabel = nullptr;  // cluster labels (allocated on first use)
   // angles
   int* aoff = nullptr;
   int2* aent = nullptr;
@@ -2248,7 +2393,7 @@ int rds_ctx_destroy(rds_ctx* c) {
   std::vector<void*> ptrs = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->d_sblocks,
                              c->upos, c->ucell, c->urank, c->count, c->cstart, c->d_step,
                              c->d_status, c->boff, c->bpart, c->br0, c->bu0, c->fbond,
-                             c->slot_of, c->aoff, c->aent, c->apart, c->aprm, c->nl4, c->ncnt, c->xref, c->scell, c->spos,
+                             c->slot_of, c->clabel, c->aoff, c->aent, c->apart, c->aprm, c->nl4, c->ncnt, c->xref, c->scell, c->spos,
                              c->stype, c->spid, c->smol};
   for (int k = 0; k < 2; ++k) {
     void* b[] = {c->buf[k].pos, c->buf[k].type, c->buf[k].type0, c->buf[k].pid, c->buf[k].mol};
@@ -2778,6 +2923,80 @@ int rds_neighbor_counts(rds_ctx* c, double r, int32_t* counts) {
   a.rprobe2 = r * r;
   if (c->has_mol) hipLaunchKernelGGL((k_count<true>), dim3((unsigned)c->n_tiles), dim3(64), c->smem_bytes, c->stream, a);
   else hipLaunchKernelGGL((k_count<false>), dim3((unsigned)c->n_tiles), dim3(64), c->smem_bytes, c->stream, a);
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+// Largest analysis radius: half the shortest periodic box length of any
+// replica (the minimum image must be unique).
+static int check_radius(rds_ctx* c, double r, const char* what) {
+  if (!(r > 0.0) || !std::isfinite(r)) return fail(RDS_E_ARG, "%s must be finite and > 0", what);
+  for (int q = 0; q < c->n_rep; ++q) {
+    const DRep& d = c->h_rep[q];
+    for (int k = 0; k < 2; ++k)
+      if ((k == 0 ? d.px : d.py) && r > 0.5 * d.L[k])
+        return fail(RDS_E_ARG, "%s %g exceeds half the box length of replica %d", what, r, q);
+  }
+  return RDS_OK;
+}
+
+int rds_pair_histogram(rds_ctx* c, double r_max, double dr, int32_t n_bins, uint32_t type_mask,
+                       uint64_t* hist) {
+  if (!c || !hist) return fail(RDS_E_ARG, "null argument");
+  if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
+  if (int e = check_radius(c, r_max, "r_max")) return e;
+  if (!(dr > 0.0) || n_bins < 1 || n_bins > 4096)
+    return fail(RDS_E_ARG, "need dr > 0 and 1 <= n_bins <= 4096");
+  Buf& cur = c->buf[c->parity];
+  HIP_TRY(hipMemsetAsync(hist, 0, sizeof(uint64_t) * n_bins * c->n_rep, c->stream));
+  // list mode: cells are those of the last rebuild, each particle within
+  // skin/2 of its cell, so the rings reach r_max + skin
+  hipLaunchKernelGGL(k_pair_hist, dim3((unsigned)c->n_sblocks), dim3(256),
+                     sizeof(unsigned) * n_bins, c->stream, c->d_rep, c->d_sblocks, c->cstart,
+                     cur.pos, cur.type, c->cpos, r_max * r_max, dr, r_max + c->skin, (int)n_bins,
+                     type_mask ? type_mask : 0xffffffffu, (unsigned long long*)hist);
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+int rds_type_counts(rds_ctx* c, int64_t* counts) {
+  if (!c || !counts) return fail(RDS_E_ARG, "null argument");
+  if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
+  Buf& cur = c->buf[c->parity];
+  HIP_TRY(hipMemsetAsync(counts, 0, sizeof(int64_t) * MAXT * c->n_rep, c->stream));
+  hipLaunchKernelGGL(k_type_counts, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
+                     c->d_sblocks, cur.type, (unsigned long long*)counts);
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
+int rds_cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* labels) {
+  if (!c || !labels) return fail(RDS_E_ARG, "null argument");
+  if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
+  if (int e = check_radius(c, r_link, "r_link")) return e;
+  if (!c->clabel && dalloc(&c->clabel, c->n_total) != RDS_OK) return RDS_E_HIP;
+  Buf& cur = c->buf[c->parity];
+  const int N = (int)c->n_total;
+  const unsigned mask = type_mask ? type_mask : 0xffffffffu;
+  hipLaunchKernelGGL(k_cl_init, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, cur.type,
+                     cur.pid, mask, c->clabel, c->slot_of);
+  // every pass that changes a label lowers one; N + 1 passes always suffice
+  for (int pass = 0;; ++pass) {
+    if (pass > N) return fail(RDS_E_STATE, "cluster labels did not converge");
+    HIP_TRY(hipMemsetAsync(&c->d_status->cl_changed, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_cl_prop, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
+                       c->d_rep, c->d_sblocks, c->cstart, cur.pos, c->cpos, r_link * r_link,
+                       r_link + c->skin, c->clabel, c->slot_of, c->d_status);
+    hipLaunchKernelGGL(k_cl_jump, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->clabel,
+                       c->slot_of);
+    int changed = 0;
+    HIP_TRY(hipMemcpyAsync(&changed, &c->d_status->cl_changed, sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!changed) break;
+  }
+  hipLaunchKernelGGL(k_cl_out, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->clabel,
+                     cur.pid, labels);
   HIP_TRY(hipGetLastError());
   return RDS_OK;
 }

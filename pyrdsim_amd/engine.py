@@ -311,6 +311,44 @@ class Engine:
         self.sync()
         return c.cpu().numpy()
 
+    # -- analysis on the device -------------------------------------------
+    @staticmethod
+    def _mask(types):
+        if types is None:
+            return 0
+        m = 0
+        for t in types:
+            m |= 1 << int(t)
+        return m
+
+    def pair_histogram(self, r_max=5.0, dr=0.02, types=None):
+        """Per replica [n_replicas][n_bins]: unordered pairs of the given types
+        (None = all) at distance r <= r_max, bin floor(r/dr); n_bins =
+        round(r_max/dr)."""
+        nb = int(round(r_max / dr))
+        h = torch.empty((len(self.replicas), nb), dtype=torch.int64, device=self.device)
+        L.check(self.lib.rds_pair_histogram(self.ctx, float(r_max), float(dr), nb,
+                                            self._mask(types), h.data_ptr()))
+        self.sync()
+        return h.cpu().numpy()
+
+    def type_counts(self):
+        """Particles of each type per replica, [n_replicas][MAX_TYPES]."""
+        cnt = torch.empty((len(self.replicas), L.MAX_TYPES), dtype=torch.int64,
+                          device=self.device)
+        L.check(self.lib.rds_type_counts(self.ctx, cnt.data_ptr()))
+        self.sync()
+        return cnt.cpu().numpy()
+
+    def cluster_labels(self, r_link=1.5, types=(1,)):
+        """Per particle (caller order): the smallest index of its cluster of
+        particles of the given types linked at r <= r_link, or -1."""
+        lab = torch.empty((self.n_total,), dtype=torch.int32, device=self.device)
+        L.check(self.lib.rds_cluster_labels(self.ctx, float(r_link), self._mask(types),
+                                            lab.data_ptr()))
+        self.sync()
+        return lab.cpu().numpy()
+
     def time_kernels(self, n_steps):
         """Average HIP-event duration (ms) of each kernel over n_steps eager steps."""
         avg = np.zeros(len(L.KERNELS), dtype=np.float64)
