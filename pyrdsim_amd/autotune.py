@@ -13,8 +13,8 @@ signature in a small JSON file.
 import json
 import os
 
-CANDIDATES = ((2, 56), (2, 64), (2, 96), (4, 56), (4, 64), (4, 80), (4, 96), (4, 112), (4, 128),
-              (4, 144))
+CANDIDATES = ((1, 56), (1, 64), (2, 56), (2, 64), (4, 48), (4, 56), (4, 64), (4, 72), (4, 80),
+              (4, 96), (4, 112), (4, 128))
 
 
 def signature(engine):
