@@ -15,7 +15,11 @@
 //                                   type gating, historic accumulation via
 //                                   the shallow dict copy at :1513)
 //   distances            :1590-1652 (minimum image with libc round)
-//   drift                :1941-2067 (interactions in file order, then bonds)
+//   drift                :1941-2067 (interactions in file order, then bonds,
+//                                   then angles)
+//   angles               :1322-1407 (straight-angle series; the flag of the
+//                                   last angle defined read by every angle;
+//                                   legs join the bonded list)
 //   reactions            :2070-2222 (rates first, then execution; p = dt*k)
 //   integrator           :2225-2250 (two separate FP64 additions)
 //   boundaries           :2253-2292 (Python-style % ; single reflection)
@@ -27,8 +31,6 @@
 // The GPU engine implements the exact semantics.
 //
 // Parity pins for every function here live in tests/test_oracle_*.py.
-// Angle potentials (:1322-1407) are not restated: parity unpinned / not
-// implemented, they are outside the engine's scope as well.
 
 #include <cmath>
 #include <cstdint>
@@ -412,7 +414,7 @@ struct Oracle {
     return 0.0;
   }
 
-  // pyx:1941-2067 (angles not restated)
+  // pyx:1941-2067
   void update_drifts() {
     std::fill(a.begin(), a.end(), 0.0);
     for (auto& it : inter) {
