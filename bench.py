@@ -458,7 +458,7 @@ def main():
             r = pick[len(pick) // 2]
             e1 = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64')
             e1.set_state(states[r][0], states[r][1])
-            plain_tiling, _ = autotune(e1, steps=8)
+            plain_tiling, _ = autotune(e1, steps=32)
             e1.close()
         for r in pick:
             e1 = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64',

@@ -1513,7 +1513,7 @@ __global__ void k_bin(int n_total, const DRep* __restrict__ rep, const int* __re
   }
 }
 
-#define SCAN_BLOCK 256
+#define SCAN_BLOCK 1024
 #define SCAN_ITEMS 8
 
 // One workgroup per replica: cell starts = pbase + exclusive prefix of the
