@@ -1249,6 +1249,52 @@ __device__ __forceinline__ double rx_rate(const DRx& rx, double rate, int ti, in
   return acc;
 }
 
+// Rates, draws and choice of one particle (pyrdsim.pyx:2103-2222): the new
+// type of particle p (type ti, neighbour counts cnt per reaction) in step
+// `step`.
+__device__ __forceinline__ int react_decide(const StepArgs& A, const DRep& Rp, const DModel& M,
+                                            int ti, int p, long long step, const int* cnt) {
+  const int nrx = M.nrx;
+  // total rate in reaction order (pyrdsim.pyx:2103-2206)
+  double total_rate = 0.0;
+#pragma unroll
+  for (int q = 0; q < MAXRX; ++q) {
+    if (q >= nrx) break;
+    rx_rate(M.rx[q], Rp.rate[q], ti, cnt[q], total_rate);
+  }
+  if (!(total_rate > 0.0)) return ti;
+  double u1, u2;
+  if (A.rng == RDS_RNG_INJECTED) {
+    long long o = step - A.inj_first;
+    if (o >= 0 && o < A.inj_count) {
+      u1 = A.inj_u1[o * A.n_total + p];
+      u2 = A.inj_u2 ? A.inj_u2[o * A.n_total + p] : 0.0;
+    } else {
+      u1 = 2.0;
+      u2 = 0.0;
+      A.status->stream = 1;
+    }
+  } else {
+    philox_uniforms(Rp, p - Rp.pbase, step, u1, u2);
+  }
+  double prob = (A.prob_mode == RDS_PROB_EXP) ? -expm1(-Rp.dt * total_rate)
+                                              : Rp.dt * total_rate;
+  if (!(prob > u1)) return ti;
+  // choice: first reaction whose cumulative share reaches u2
+  // (pyrdsim.pyx:2215-2222); the rates are recomputed bit-identically
+  double sacc = 0.0, dummy = 0.0;
+  int chosen = -1;
+#pragma unroll
+  for (int q = 0; q < MAXRX; ++q) {
+    if (q >= nrx || chosen >= 0) break;
+    sacc += rx_rate(M.rx[q], Rp.rate[q], ti, cnt[q], dummy) / total_rate;
+    if (sacc >= u2) chosen = q;
+  }
+  if (chosen < 0) return ti;
+  atomicAdd(&A.status->events, 1ull);
+  return M.rx[chosen].product;
+}
+
 template <bool MOL>
 __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
   const int4 B = A.sblocks[blockIdx.x];  // (replica, first slot, end slot)
@@ -1342,47 +1388,7 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
         }
       }
     }
-    // total rate in reaction order (pyrdsim.pyx:2103-2206)
-    double total_rate = 0.0;
-#pragma unroll
-    for (int q = 0; q < MAXRX; ++q) {
-      if (q >= nrx) break;
-      rx_rate(M.rx[q], Rp.rate[q], ti, cnt[q], total_rate);
-    }
-    if (total_rate > 0.0) {
-      double u1, u2;
-      if (A.rng == RDS_RNG_INJECTED) {
-        long long o = step - A.inj_first;
-        if (o >= 0 && o < A.inj_count) {
-          u1 = A.inj_u1[o * A.n_total + p];
-          u2 = A.inj_u2 ? A.inj_u2[o * A.n_total + p] : 0.0;
-        } else {
-          u1 = 2.0;
-          u2 = 0.0;
-          A.status->stream = 1;
-        }
-      } else {
-        philox_uniforms(Rp, p - Rp.pbase, step, u1, u2);
-      }
-      double prob = (A.prob_mode == RDS_PROB_EXP) ? -expm1(-Rp.dt * total_rate)
-                                                  : Rp.dt * total_rate;
-      if (prob > u1) {
-        // choice: first reaction whose cumulative share reaches u2
-        // (pyrdsim.pyx:2215-2222); the rates are recomputed bit-identically
-        double sacc = 0.0, dummy = 0.0;
-        int chosen = -1;
-#pragma unroll
-        for (int q = 0; q < MAXRX; ++q) {
-          if (q >= nrx || chosen >= 0) break;
-          sacc += rx_rate(M.rx[q], Rp.rate[q], ti, cnt[q], dummy) / total_rate;
-          if (sacc >= u2) chosen = q;
-        }
-        if (chosen >= 0) {
-          tnew = M.rx[chosen].product;
-          atomicAdd(&A.status->events, 1ull);
-        }
-      }
-    }
+    tnew = react_decide(A, Rp, M, ti, p, step, cnt);
   }
   A.type_out[s] = tnew;
 }
