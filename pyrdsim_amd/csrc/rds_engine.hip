@@ -1060,7 +1060,21 @@ void k_force(const StepArgs A, int mode, int pw) {
       const int cyy = cell_coord(v[1], Rp.cl[1], Rp.ly);
       const int c = Rp.cbase + cyy * Rp.lx + cx;
       A.ucell[s] = c;
-      A.urank[s] = atomicAdd(&A.count[c], 1);
+      // ranks in the new cells, one atomic per run of lanes with the same new
+      // cell: the lanes of a batch are a prefix of the wave (the valid home
+      // particles), dealt cell by cell, and most particles stay in their cell
+      const unsigned long long act = __ballot(1);
+      const int nact = __popcll(act);
+      const int cprev = __shfl_up(c, 1, 64);
+      const bool head = lane == 0 || cprev != c;
+      const unsigned long long heads = __ballot(head);
+      const unsigned long long below = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+      const int hl = 63 - __clzll(heads & below);          // this run's first lane
+      const unsigned long long above = heads & ~below;
+      const int next = above ? min(__ffsll((long long)above) - 1, nact) : nact;
+      int base = 0;
+      if (head) base = atomicAdd(&A.count[c], next - lane);
+      A.urank[s] = __shfl(base, hl, 64) + (lane - hl);
     }
   }
 }
