@@ -345,9 +345,9 @@ def main():
           file=sys.stderr, flush=True)
     traffic = None
     # the counter passes use the tile shape the autotuner picks for this
-    # workload (4 rows x 64 particles), unless one is given
+    # workload (4 rows x 112 particles), unless one is given
     traffic_tiling = (args.tile_rows if args.tile_rows > 1 else 4,
-                      args.tile_particles if args.tile_particles > 0 else 64)
+                      args.tile_particles if args.tile_particles > 0 else 112)
     if args.traffic:
         traffic = measure_traffic(cache, traffic_tiling[0], traffic_tiling[1], args.replicas)
         print('# traffic: %s' % traffic, file=sys.stderr, flush=True)

@@ -729,13 +729,14 @@ __device__ __forceinline__ bool in_range(double r2, double rc2) {
 // pair differences and the update are float arithmetic).
 // Waves per SIMD the register budget of a force kernel must allow: the
 // register-resident LJ path on the cell stencil, in the 4-row tiles that
-// large batches use, is latency-bound and gains from occupancy (<= 64 VGPRs
-// for FP32 pair arithmetic, <= 72 for FP64, a few spilled registers in the
-// epilogue).  Smaller grids gain nothing from it and keep the compiler's
-// choice, as do the other paths.
+// large batches use, gains from occupancy (<= 72 VGPRs, 7 waves per SIMD,
+// with FP32 pair arithmetic; <= 80, 6 waves, with FP64).  Smaller grids gain
+// nothing from it and keep the compiler's choice, as do the other paths.
+#define FORCE_WAVES32 7
+#define FORCE_WAVES64 6
 template <typename Real, int KM, int R, bool LIST>
 constexpr int force_waves() {
-  return (KM == 0 || LIST || R != 4) ? 1 : (sizeof(Real) == 4 ? 8 : 7);
+  return (KM == 0 || LIST || R != 4) ? 1 : (sizeof(Real) == 4 ? FORCE_WAVES32 : FORCE_WAVES64);
 }
 
 template <typename Real, bool MOL, int KM, int R, bool LIST, int PM>
@@ -2371,11 +2372,12 @@ bool build_tiles(rds_ctx* c, int rows, int target) {
   c->n_tiles = (int64_t)c->h_tiles.size();
   const int per = 16 + 4 + (c->has_mol ? 4 : 0);
   // 1.5x headroom, trimmed (not below 1.15x) where it would cost the force
-  // kernel occupancy: 32 waves per CU of the register-resident path leave
-  // 160 KB / (32 / rows) of LDS per workgroup
+  // kernel occupancy: 4 x waves-per-SIMD waves per CU of the register-resident
+  // path leave 160 KB / (4 waves / rows) of LDS per workgroup
   const size_t tile_sh = rows == 4 ? sizeof(TileSh<4>) : rows == 2 ? sizeof(TileSh<2>)
                                                                    : sizeof(TileSh<1>);
-  const long budget = (long)(160 * 1024 / (32 / rows)) - (long)tile_sh - 64;
+  const int waves = c->precision == RDS_FP64 ? FORCE_WAVES64 : FORCE_WAVES32;
+  const long budget = (long)(160 * 1024 * rows / (4 * waves)) - (long)tile_sh - 64;
   const int cap_occ = budget > 0 ? (int)(budget / per / 64 * 64) : 0;
   int cap = (int)std::ceil(1.5 * staged_worst / 64.0) * 64;
   cap = std::max((int)std::ceil(1.15 * staged_worst / 64.0) * 64, std::min(cap, cap_occ));
