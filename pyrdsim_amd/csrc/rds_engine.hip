@@ -728,15 +728,19 @@ __device__ __forceinline__ bool in_range(double r2, double rc2) {
 // PM: 0 = FP64 positions, 1 = compensated FP32 positions (float pairs; the
 // pair differences and the update are float arithmetic).
 // Waves per SIMD the register budget of a force kernel must allow: the
-// register-resident LJ path on the cell stencil, in the 4-row tiles that
-// large batches use, gains from occupancy (<= 72 VGPRs, 7 waves per SIMD,
-// with FP32 pair arithmetic; <= 80, 6 waves, with FP64).  Smaller grids gain
-// nothing from it and keep the compiler's choice, as do the other paths.
+// register-resident LJ path, in the 4-row tiles that large batches use,
+// gains from occupancy (cell stencil: <= 72 VGPRs, 7 waves per SIMD, with
+// FP32 pair arithmetic and <= 80, 6 waves, with FP64; lists: 6 waves).
+// Smaller grids gain nothing from it and keep the compiler's choice, as does
+// the general term table.
 #define FORCE_WAVES32 7
 #define FORCE_WAVES64 6
+#define FORCE_WAVES_LIST 6
 template <typename Real, int KM, int R, bool LIST>
 constexpr int force_waves() {
-  return (KM == 0 || LIST || R != 4) ? 1 : (sizeof(Real) == 4 ? FORCE_WAVES32 : FORCE_WAVES64);
+  return (KM == 0 || R != 4) ? 1
+         : LIST              ? FORCE_WAVES_LIST
+                             : (sizeof(Real) == 4 ? FORCE_WAVES32 : FORCE_WAVES64);
 }
 
 template <typename Real, bool MOL, int KM, int R, bool LIST, int PM>
