@@ -3,7 +3,10 @@
 Headline (`value`): particle-steps/s of the batched ensemble -- 512 independent
 replicas (256 of the two-species LJ mixture, 256 of the wall + reaction model,
 N per replica log-uniform over 2,048..32,768, sum N = 5,789,872) stepped
-together with FP32 pair forces and FP64 positions, hipGraph-captured.  Beside it:
+together with FP32 pair forces and FP64 positions, hipGraph-captured.  The
+ensemble is first advanced --pre-advance steps (default 2,000, untimed) from
+its all-inactive start into its reacting regime; the timed window replays
+graphs of at most --steps steps.  Beside it:
 
 * `plain`: the single-trajectory FP64 path, one replica at a time, on a
   stratified subset of the same replicas, and the ratio batched / plain;
@@ -12,16 +15,18 @@ together with FP32 pair forces and FP64 positions, hipGraph-captured.  Beside it
   against its algorithmic HBM bytes (48 B per particle + 4 B per cell);
 * `cpu_baseline`: the CPU oracle in reference-faithful mode (Verlet list on the
   reference's grid, FP64, one core) on a bounded sample of the same replicas;
+  `c1`: the one-species LJ system (N = 1,024, 2,000 steps, FP64) on the GPU
+  and on the CPU oracle (median of 3);
 * `analysis`: the RDF pair histogram, type counts and cluster labels of the
-  whole ensemble on the device, beside the host RDF of one replica.
+  whole ensemble on the device, beside the host RDF of one replica;
+* `steady_state`: 300 further graphed steps timed right after the window, as
+  a check that the window is representative.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 
 import argparse
 import json
-import math
-import multiprocessing as mp
 import os
 import sys
 import time
@@ -31,7 +36,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from pyrdsim_amd import workloads  # noqa: E402
+from pyrdsim_amd import ensemble, workloads  # noqa: E402
 from pyrdsim_amd.autotune import autotune  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -41,60 +46,6 @@ FORCE_BYTES_PER_CELL = 4        # cell starts
 # write pos 16; plus 2 bytes per list entry
 LIST_BYTES_PER_PARTICLE = 62
 LIST_BYTES_PER_ENTRY = 2
-
-
-def _make_replica_state(args):
-    kind, n, k_a, k_d, seed, ic_seed = args
-    if kind == 'kalj':
-        s = workloads.kalj(n=n, seed=ic_seed)
-    else:
-        s = workloads.wall(n=n, seed=ic_seed, k_a=k_a, k_d=k_d)
-    return s['x'], s['types']
-
-
-def build_ensemble(n_rep, workers, cache=None):
-    """Replica plan and initial states; `cache` (an .npz path) is read when it
-    exists and written otherwise, so that profiled runs start without forking
-    worker processes."""
-    plan = workloads.ensemble_plan(n_rep=n_rep)
-    if cache and os.path.exists(cache):
-        with np.load(cache, allow_pickle=False) as z:
-            x, t, n = z['x'], z['t'], z['n']
-        if len(n) == len(plan) and all(int(a) == p['n'] for a, p in zip(n, plan)):
-            off = np.concatenate([[0], np.cumsum(n)])
-            return plan, [(x[off[r]:off[r + 1]], t[off[r]:off[r + 1]]) for r in range(len(plan))]
-    jobs = [(p['kind'], p['n'], p['k_a'], p['k_d'], p['seed'], 1005 * 1000 + r)
-            for r, p in enumerate(plan)]
-    if workers > 1:
-        ctx = mp.get_context('fork')
-        with ctx.Pool(workers) as pool:
-            states = pool.map(_make_replica_state, jobs, chunksize=4)
-    else:
-        states = [_make_replica_state(j) for j in jobs]
-    if cache:
-        np.savez(cache, x=np.concatenate([s[0] for s in states]),
-                 t=np.concatenate([s[1] for s in states]),
-                 n=np.array([p['n'] for p in plan]))
-    return plan, states
-
-
-def engine_pieces(plan):
-    from pyrdsim_amd.engine import build_model, make_replica
-    mk = workloads.kalj_model()
-    mw = workloads.wall_model()
-    kalj_model, _ = build_model(mk['interactions'], mk['reactions'], 2)
-    wall_model, _ = build_model(mw['interactions'], mw['reactions'], 2)
-    L_k = lambda n: workloads.square_box(n, 0.6)
-    L_w = lambda n: workloads.square_box(n, 0.4)
-    reps = []
-    for p in plan:
-        if p['kind'] == 'kalj':
-            reps.append(make_replica(0, p['n'], L_k(p['n']), [1, 1], 5e-5, [1.0, 1.0], [],
-                                     p['seed']))
-        else:
-            reps.append(make_replica(1, p['n'], L_w(p['n']), [1, -1], 1e-4, [1.0, 1.0],
-                                     [p['k_a'], p['k_d']], p['seed']))
-    return [kalj_model, wall_model], reps
 
 
 def time_engine(eng, steps, warmup):
@@ -143,7 +94,7 @@ def accuracy_leg(plan, states, models, reps, x0, t0s, graph_steps, tiling, steps
 
     ref = {}
     for r in picks:
-        e = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64',
+        e = Engine([models[reps[r].model]], [ensemble.with_model0(reps[r])], precision='fp64',
                    graph_steps=graph_steps)
         e.set_state(states[r][0], states[r][1])
         for k in range(n_chunks):
@@ -217,13 +168,14 @@ def analysis_leg(eng, plan, reps):
                 host_sample='analysis.rdf on replica %d (N=%d), one core' % (r, int(n[r])))
 
 
-def measure_traffic(cache, rows, per, replicas, timeout=300):
+def measure_traffic(cache, state, rows, per, replicas, timeout=300):
     """HBM bytes per force-kernel launch from two rocprofv3 counter passes
-    (FETCH_SIZE, then WRITE_SIZE) over a few eager steps of this workload in a
-    child process, each scaled by what the same counter reads for an
-    elementwise kernel that reads and writes 256 MiB (--copy-calibration).
-    Run before this process touches the GPU.  None when rocprofv3 is missing
-    or a pass fails."""
+    (FETCH_SIZE, then WRITE_SIZE) over this workload in a child process that
+    starts from `state` (the main run's advanced state, saved by
+    save_state): the mean over its last eager k_force launches, each pass
+    scaled by what the same counter reads for an elementwise kernel that
+    reads and writes 256 MiB (--copy-calibration).  None when rocprofv3 is
+    missing or a pass fails."""
     import csv
     import glob
     import shutil
@@ -233,13 +185,16 @@ def measure_traffic(cache, rows, per, replicas, timeout=300):
     if not prof:
         return None
     tmp = tempfile.mkdtemp(prefix='rds_traffic_')
-    child = [sys.executable, os.path.abspath(__file__), '--steps', '6', '--warmup', '2',
-             '--graph-steps', '0', '--replicas', str(replicas), '--no-plain', '--no-cpu',
-             '--no-accuracy', '--no-autotune', '--no-traffic', '--tile-rows', str(rows),
-             '--tile-particles', str(per), '--copy-calibration', '--ic-cache', cache]
+    child = [sys.executable, os.path.abspath(__file__), '--steps', '6', '--warmup', '0',
+             '--graph-steps', '0', '--pre-advance', '0', '--replicas', str(replicas),
+             '--no-plain', '--no-cpu', '--no-accuracy', '--no-autotune', '--no-traffic',
+             '--no-analysis', '--no-c1', '--no-steady-state', '--tile-rows', str(rows),
+             '--tile-particles', str(per), '--copy-calibration', '--ic-cache', cache,
+             '--state', state]
     out = {}
     try:
         for counter in ('FETCH_SIZE', 'WRITE_SIZE'):
+            print('# counter pass %s' % counter, file=sys.stderr, flush=True)
             d = os.path.join(tmp, counter)
             cmd = [prof, '--pmc', counter, '--output-format', 'csv', '-d', d, '-o', 'c', '--'] + child
             subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
@@ -250,6 +205,7 @@ def measure_traffic(cache, rows, per, replicas, timeout=300):
             with open(files[0]) as f:
                 recs = [r for r in csv.DictReader(f) if r['Counter_Name'] == counter]
             force = [float(r['Counter_Value']) for r in recs if 'k_force' in r['Kernel_Name']]
+            force = force[-16:]
             cal = [float(r['Counter_Value']) for r in recs if 'AUnaryFunctor' in r['Kernel_Name']]
             if not force or not cal or cal[0] <= 0:
                 return None
@@ -264,7 +220,8 @@ def measure_traffic(cache, rows, per, replicas, timeout=300):
 
 def cpu_baseline(plan, states, budget_s):
     """The oracle, reference-faithful (Verlet list, rs = 2 rc, ratio 2, refresh
-    every 10 steps, FP64, one core), on the smallest replica of each model."""
+    every 10 steps, FP64, one core), on the smallest replica of each model of
+    the ensemble, about budget_s seconds in all."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     from oracle import Oracle
     picks = []
@@ -277,14 +234,11 @@ def cpu_baseline(plan, states, budget_s):
     for r in picks:
         p = plan[r]
         x, t = states[r]
-        if p['kind'] == 'kalj':
-            m = workloads.kalj_model()
-            L, per, dt = workloads.square_box(p['n'], 0.6), [1, 1], 5e-5
-        else:
-            m = workloads.wall_model(p['k_a'], p['k_d'])
-            L, per, dt = workloads.square_box(p['n'], 0.4), [1, -1], 1e-4
-        o = Oracle(x, t, L, per, dt, m['tau_by_tauD'], m['interactions'], m['reactions'],
-                   seed=p['seed'], rng='mt19937', nl='faithful', ratio=2, nl_freq=10)
+        sp = ensemble.replica_spec(p)
+        m = sp['model']
+        o = Oracle(x, t, sp['L'], sp['periodic'], sp['dt'], m['tau_by_tauD'], m['interactions'],
+                   m['reactions'], seed=p['seed'], rng='mt19937', nl='faithful', ratio=2,
+                   nl_freq=10)
         o.init()
         steps = 0
         t0 = time.perf_counter()
@@ -298,21 +252,99 @@ def cpu_baseline(plan, states, budget_s):
     return done / t_total, '; '.join(desc) + ' (%.1f s)' % t_total
 
 
+def c1_leg(runs=3, steps=2000):
+    """The one-species LJ system (N = 1,024, density 0.5, rc = 2.5, dt = 1e-4,
+    2,000 steps, FP64): the plain GPU path (graphs of 50 steps, median of
+    `runs`) and the oracle in reference-faithful mode on one core (median of
+    `runs`), in particle-steps/s."""
+    import torch
+    from pyrdsim_amd.engine import Engine, build_model, make_replica
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    from oracle import Oracle
+    s = workloads.lj1()
+    m = s['model']
+    n = len(s['x'])
+    model, rates = build_model(m['interactions'], m['reactions'], 1)
+    gpu = []
+    for k in range(runs):
+        rep = make_replica(0, n, s['L'], s['periodic'], s['dt'], m['tau_by_tauD'], rates, 2001)
+        e = Engine([model], [rep], precision='fp64', graph_steps=50)
+        e.set_state(s['x'], s['types'])
+        gpu.append(n * steps / time_engine(e, steps, 50))
+        e.close()
+    torch.cuda.synchronize()
+    cpu = []
+    for k in range(runs):
+        o = Oracle(s['x'], s['types'], s['L'], s['periodic'], s['dt'], m['tau_by_tauD'],
+                   m['interactions'], m['reactions'], seed=2001, rng='mt19937',
+                   nl='faithful', ratio=2, nl_freq=10)
+        o.init()
+        t0 = time.perf_counter()
+        o.run(1, steps)
+        cpu.append(n * steps / (time.perf_counter() - t0))
+    return dict(workload='one-species LJ, N=1024, rho=0.5, rc=2.5, dt=1e-4, %d steps, FP64'
+                % steps, gpu_plain=float(np.median(gpu)),
+                cpu_oracle_faithful_1core=float(np.median(cpu)), runs=runs,
+                unit='particle-steps/s')
+
+
+def copy_bandwidth(nbytes=2 << 30, reps=10):
+    """Streaming copy of nbytes between two device buffers (torch's copy
+    kernel), timed with HIP events: (read + write bytes) / time in GB/s, the
+    best of `reps`."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device='cuda').fill_(1.0)
+    b = torch.empty_like(a)
+    best = 0.0
+    for _ in range(reps + 1):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        best = max(best, 2 * nbytes / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del a, b
+    torch.cuda.synchronize()
+    return best
+
+
+def save_state(eng, path):
+    """Positions, types and step counter of an engine, for a child process."""
+    x, t = eng.state()
+    np.savez(path, x=x, t=t, step=np.int64(eng.step))
+
+
+def advance(eng, n):
+    """n untimed steps (graph replays where the engine has graphs)."""
+    if n > 0:
+        eng.run(n)
+        eng.sync()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=500)
     ap.add_argument('--warmup', type=int, default=50)
+    ap.add_argument('--pre-advance', type=int, default=2000,
+                    help='untimed steps from the all-inactive start into the reacting regime '
+                         'before any timed window')
     ap.add_argument('--replicas', type=int, default=512)
     ap.add_argument('--plain-subset', type=int, default=32)
     ap.add_argument('--plain-steps', type=int, default=500)
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
-    ap.add_argument('--graph-steps', type=int, default=50)
+    ap.add_argument('--graph-steps', type=int, default=50,
+                    help='steps per captured graph (at most --steps, so that the timed '
+                         'window replays graphs)')
     ap.add_argument('--tile-particles', type=int, default=0)
     ap.add_argument('--tile-rows', type=int, default=1)
     ap.add_argument('--skin', type=float, default=0.0,
                     help='Verlet skin of the batched engine (0: cell search every step)')
     ap.add_argument('--ic-cache', default=None, help='.npz file of initial states (read or written)')
+    ap.add_argument('--state', default=None,
+                    help='.npz file (x, t, step) to start the batched run from instead of the '
+                         'initial states (the counter passes use it)')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-autotune', dest='autotune', action='store_false')
     ap.add_argument('--no-accuracy', action='store_true')
@@ -322,7 +354,9 @@ def main():
                     help='run one elementwise kernel of known traffic (256 MiB read + 256 MiB '
                          'written) first, to scale HBM counters in a profile of this command')
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-c1', action='store_true')
     ap.add_argument('--no-analysis', action='store_true')
+    ap.add_argument('--no-steady-state', action='store_true')
     ap.add_argument('--no-traffic', dest='traffic', action='store_false',
                     help='skip the rocprofv3 counter passes for roofline.traffic')
     args = ap.parse_args()
@@ -330,6 +364,7 @@ def main():
         print('bench.py runs on one GPU only: this build has no multi-GPU path '
               '(--gpus %d requested)' % args.gpus, file=sys.stderr)
         sys.exit(2)
+    graph_steps = max(0, min(args.graph_steps, args.steps))
 
     t0 = time.perf_counter()
     cache = args.ic_cache
@@ -340,19 +375,10 @@ def main():
         os.close(fd)
         os.unlink(own_cache)
         cache = own_cache
-    plan, states = build_ensemble(args.replicas, min(16, os.cpu_count() or 1), cache)
+    plan = workloads.ensemble_plan(n_rep=args.replicas)
+    states = ensemble.states(plan, min(16, os.cpu_count() or 1), cache)
     print('# built %d replica states in %.1f s' % (len(plan), time.perf_counter() - t0),
           file=sys.stderr, flush=True)
-    traffic = None
-    # the counter passes use the tile shape the autotuner picks for this
-    # workload (4 rows x 112 particles), unless one is given
-    traffic_tiling = (args.tile_rows if args.tile_rows > 1 else 4,
-                      args.tile_particles if args.tile_particles > 0 else 112)
-    if args.traffic:
-        traffic = measure_traffic(cache, traffic_tiling[0], traffic_tiling[1], args.replicas)
-        print('# traffic: %s' % traffic, file=sys.stderr, flush=True)
-    if own_cache and os.path.exists(own_cache):
-        os.unlink(own_cache)
 
     import torch
     from pyrdsim_amd.engine import Engine
@@ -363,29 +389,55 @@ def main():
         torch.cuda.synchronize()
         del a, b
         print('# copy calibration: 268435456 B read + 268435456 B written', file=sys.stderr)
-    models, reps = engine_pieces(plan)
+    copy_gbs = copy_bandwidth()
+    print('# streaming copy: %.0f GB/s' % copy_gbs, file=sys.stderr, flush=True)
+    models, reps = ensemble.engine_inputs(plan)
     n_tot = sum(p['n'] for p in plan)
     x0 = np.concatenate([s[0] for s in states])
     t0s = np.concatenate([s[1] for s in states])
 
-    eng = Engine(models, reps, precision='mixed', graph_steps=args.graph_steps,
+    eng = Engine(models, reps, precision='mixed', graph_steps=graph_steps,
                  tile_particles=args.tile_particles, tile_rows=args.tile_rows, skin=args.skin)
-    eng.set_state(x0, t0s)
+    step0 = 0
+    if args.state:
+        with np.load(args.state, allow_pickle=False) as z:
+            x0, t0s, step0 = z['x'], z['t'], int(z['step'])
+    eng.set_state(x0, t0s, step0=step0)
     tuned = None
     if args.autotune:
         best, times = autotune(eng)
         tuned = dict(best=list(best), force_ms={'%dx%d' % k: round(v, 5) for k, v in times.items()})
         print('# autotune: rows x particles = %s' % (best,), file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    advance(eng, args.pre_advance)
+    print('# pre-advanced %d steps in %.1f s' % (args.pre_advance, time.perf_counter() - t),
+          file=sys.stderr, flush=True)
     st0 = eng.sync()
     secs = time_engine(eng, args.steps, args.warmup)
     st1 = eng.sync()
     value = n_tot * args.steps / secs
     print('# batched: %.3e particle-steps/s (%.3f ms/step)' % (value, 1e3 * secs / args.steps),
           file=sys.stderr, flush=True)
+    steady = None
+    if not args.no_steady_state:
+        ss_steps = 300
+        ss = time_engine(eng, ss_steps, 0)
+        steady = dict(steps=ss_steps, ms_per_step=round(1e3 * ss / ss_steps, 5),
+                      first_step=int(args.pre_advance + args.warmup + args.steps + 1),
+                      graph_steps=graph_steps)
+        print('# steady state: %s' % steady, file=sys.stderr, flush=True)
 
     kt = eng.time_kernels(20)
     eng.sync()
     st2 = eng.sync()
+    counts = eng.type_counts()
+    state_file = None
+    if args.traffic:
+        import tempfile
+        fd, state_file = tempfile.mkstemp(prefix='rds_state_', suffix='.npz')
+        os.close(fd)
+        save_state(eng, state_file)
+    act = [counts[r, 1] / p['n'] for r, p in enumerate(plan) if p['kind'] == 'wall']
     _, n_cells, n_tiles = eng.sizes()
     force_ms = kt['force'][0]
     lists = None
@@ -400,6 +452,25 @@ def main():
     else:
         force_bytes = FORCE_BYTES_PER_PARTICLE * n_tot + FORCE_BYTES_PER_CELL * n_cells
     achieved = force_bytes / (force_ms * 1e-3) / 1e9
+    tiling = eng.tiling()
+    ana = None
+    if not args.no_analysis:
+        ana = analysis_leg(eng, plan, reps)
+        print('# analysis: %s' % ana, file=sys.stderr, flush=True)
+    del eng
+    torch.cuda.synchronize()
+    traffic = None
+    # the counter passes use this run's tile shape
+    traffic_tiling = tuple(tiling[:2])
+    if args.traffic:
+        t = time.perf_counter()
+        traffic = measure_traffic(cache, state_file, traffic_tiling[0], traffic_tiling[1],
+                                  args.replicas)
+        print('# traffic: %s (%.1f s)' % (traffic, time.perf_counter() - t), file=sys.stderr,
+              flush=True)
+        os.unlink(state_file)
+    if own_cache and os.path.exists(own_cache):
+        os.unlink(own_cache)
     traffic_bytes = int(traffic['FETCH_SIZE'] + traffic['WRITE_SIZE']) if traffic else None
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_bytes,
@@ -408,38 +479,31 @@ def main():
                     traffic_tiling=list(traffic_tiling) if traffic else None,
                     kernel='k_force (pair forces + Euler-Maruyama + binning)',
                     algorithmic_bytes=int(force_bytes),
-                    avg_ms={k: round(v[0], 5) for k, v in kt.items() if v[1]})
-    tiling = eng.tiling()
-    ana = None
-    if not args.no_analysis:
-        ana = analysis_leg(eng, plan, reps)
-        print('# analysis: %s' % ana, file=sys.stderr, flush=True)
-    del eng
-    torch.cuda.synchronize()
+                    copy_measured=round(copy_gbs, 1),
+                    frac_of_measured_copy=round(achieved / copy_gbs, 4),
+                    avg_ms={k: round(v[0], 5) for k, v in kt.items() if v[1]},
+                    at_step=int(args.pre_advance + args.warmup + args.steps +
+                                (0 if steady is None else steady['steps'])))
 
     accuracy = None
     compensated = None
     batched_fp64 = None
     if not args.no_accuracy:
-        # throughput of the same batched run with compensated FP32 positions
-        ec = Engine(models, reps, precision='compensated', graph_steps=args.graph_steps,
-                    tile_rows=tiling[0], tile_particles=tiling[1])
-        ec.set_state(x0, t0s)
-        secs_c = time_engine(ec, args.steps, args.warmup)
-        compensated = dict(value=n_tot * args.steps / secs_c, unit='particle-steps/s',
-                           ms_per_step=1e3 * secs_c / args.steps)
-        del ec
-        torch.cuda.synchronize()
-        # the same batched run in FP64: separates the gain of batching from
-        # that of FP32 pair arithmetic
-        e64 = Engine(models, reps, precision='fp64', graph_steps=args.graph_steps,
-                     tile_rows=tiling[0], tile_particles=tiling[1])
-        e64.set_state(x0, t0s)
-        secs_64 = time_engine(e64, args.steps, args.warmup)
-        batched_fp64 = dict(value=n_tot * args.steps / secs_64, unit='particle-steps/s',
-                            ms_per_step=1e3 * secs_64 / args.steps)
-        del e64
-        torch.cuda.synchronize()
+        # throughput of the same batched run with compensated FP32 positions,
+        # and in FP64 (separates the gain of batching from that of FP32 pair
+        # arithmetic), each pre-advanced like the headline run
+        legs = {}
+        for prec in ('compensated', 'fp64'):
+            e = Engine(models, reps, precision=prec, graph_steps=graph_steps,
+                       tile_rows=tiling[0], tile_particles=tiling[1])
+            e.set_state(x0, t0s)
+            advance(e, args.pre_advance)
+            sec = time_engine(e, args.steps, args.warmup)
+            legs[prec] = dict(value=n_tot * args.steps / sec, unit='particle-steps/s',
+                              ms_per_step=1e3 * sec / args.steps)
+            del e
+            torch.cuda.synchronize()
+        compensated, batched_fp64 = legs['compensated'], legs['fp64']
         accuracy = accuracy_leg(plan, states, models, reps, x0, t0s, args.graph_steps, tiling,
                                 args.accuracy_steps, args.accuracy_replicas)
         print('# accuracy: %s' % accuracy, file=sys.stderr, flush=True)
@@ -456,29 +520,36 @@ def main():
         if args.autotune:
             # tuned once, on the median-size replica of the subset
             r = pick[len(pick) // 2]
-            e1 = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64')
+            e1 = Engine([models[reps[r].model]], [ensemble.with_model0(reps[r])], precision='fp64')
             e1.set_state(states[r][0], states[r][1])
             plain_tiling, _ = autotune(e1, steps=32)
             e1.close()
+        plain_graph = min(args.graph_steps, args.plain_steps)
         for r in pick:
-            e1 = Engine([models[reps[r].model]], [_with_model0(reps[r])], precision='fp64',
-                        graph_steps=args.graph_steps, tile_rows=plain_tiling[0],
+            e1 = Engine([models[reps[r].model]], [ensemble.with_model0(reps[r])], precision='fp64',
+                        graph_steps=plain_graph, tile_rows=plain_tiling[0],
                         tile_particles=plain_tiling[1])
             e1.set_state(states[r][0], states[r][1])
+            advance(e1, args.pre_advance)
             dt = time_engine(e1, args.plain_steps, 20)
             work += plan[r]['n'] * args.plain_steps
             tsum += dt
             e1.close()
         plain = dict(value=work / tsum, unit='particle-steps/s', replicas=k,
-                     steps=args.plain_steps, precision='fp64', graph_steps=args.graph_steps,
-                     tiling=list(plain_tiling))
+                     steps=args.plain_steps, precision='fp64', graph_steps=plain_graph,
+                     tiling=list(plain_tiling), pre_advance=args.pre_advance)
         print('# plain: %.3e particle-steps/s' % plain['value'], file=sys.stderr, flush=True)
 
+    c1 = None
+    if not args.no_c1:
+        c1 = c1_leg()
+        print('# c1: %s' % c1, file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         v, desc = cpu_baseline(plan, states, args.cpu_seconds)
         cpu = dict(value=v, unit='particle-steps/s', cores=1, kind='port',
-                   sample='oracle, reference-faithful FP64: ' + desc)
+                   sample='oracle, reference-faithful FP64, mt19937 stream: ' + desc,
+                   c1=c1)
 
     out = {
         'metric': 'particle-steps/s (batched vs plain, 1 MI355X) at RDF error <=1%; % HBM roofline',
@@ -497,10 +568,15 @@ def main():
             'workload': 'ensemble: %d replicas (half LJ mixture rho=0.6 dt=5e-5, half '
                         'wall+reactions rho=0.4 dt=1e-4), N log-uniform 2048..32768, '
                         'sum N=%d' % (len(plan), n_tot),
-            'precision': 'mixed', 'graph_steps': args.graph_steps,
+            'precision': 'mixed', 'graph_steps': graph_steps,
+            'pre_advance': args.pre_advance,
+            'timed_steps': [int(args.pre_advance + args.warmup + 1),
+                            int(args.pre_advance + args.warmup + args.steps)],
+            'activated_fraction_wall_mean': round(float(np.mean(act)), 5) if act else None,
             'tiling': list(tiling[:2]), 'lds_cap': tiling[2], 'autotune': tuned,
             'cells': int(n_cells), 'tiles': int(n_tiles), 'lists': lists},
         'roofline': roofline,
+        'steady_state': steady,
         'plain': plain,
         'batched_vs_plain': (value / plain['value']) if plain else None,
         'accuracy': accuracy,
@@ -510,15 +586,6 @@ def main():
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))
-
-
-def _with_model0(rep):
-    import ctypes
-    from pyrdsim_amd import _lib as L
-    r = L.Replica()
-    ctypes.pointer(r)[0] = rep
-    r.model = 0
-    return r
 
 
 if __name__ == '__main__':

@@ -79,7 +79,7 @@ def load():
         'orc_set_angles': (C.c_int, [vp, C.c_int64, vp, vp, vp, vp, vp, C.c_double]),
         'orc_run': (C.c_int, [vp, C.c_int64, C.c_int64]),
         'orc_get_state': (C.c_int, [vp, vp, vp, vp]),
-        'orc_set_state': (C.c_int, [vp, vp, vp]),
+        'orc_set_state': (C.c_int, [vp, vp, vp, vp]),
         'orc_reaction_info': (C.c_int64, [vp, vp, vp]),
         'orc_nl_size': (C.c_int64, [vp]),
         'orc_nl_pairs': (C.c_int64, [vp, vp, C.c_int64]),
@@ -262,10 +262,14 @@ class Oracle:
         load().orc_get_state(self.h, x.ctypes.data, t.ctypes.data, a.ctypes.data)
         return x, t, a
 
-    def set_state(self, x, types):
+    def set_state(self, x, types, drift=None):
+        """Replace positions and types (and the drift of the last step); the
+        neighbour list stays as it is until its next refresh."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         t = np.ascontiguousarray(types, dtype=np.int32)
-        load().orc_set_state(self.h, x.ctypes.data, t.ctypes.data)
+        a = None if drift is None else np.ascontiguousarray(drift, dtype=np.float64)
+        load().orc_set_state(self.h, x.ctypes.data, t.ctypes.data,
+                             None if a is None else a.ctypes.data)
 
     def reaction_info(self):
         rates = np.zeros((max(1, self.n_rx), self.n))

@@ -731,10 +731,13 @@ int orc_get_state(void* h, double* x, int32_t* types, double* drift) {
   return 0;
 }
 
-int orc_set_state(void* h, const double* x, const int32_t* types) {
+// Replaces positions, types and the drift of the last step (each optional);
+// the neighbour list is kept as it is until its next refresh.
+int orc_set_state(void* h, const double* x, const int32_t* types, const double* drift) {
   Oracle* o = (Oracle*)h;
   if (x) std::memcpy(o->x.data(), x, o->x.size() * sizeof(double));
   if (types) std::memcpy(o->types.data(), types, o->types.size() * sizeof(int32_t));
+  if (drift) std::memcpy(o->a.data(), drift, o->a.size() * sizeof(double));
   return 0;
 }
 

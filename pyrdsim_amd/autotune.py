@@ -5,9 +5,11 @@ particles dealt to the wavefronts 64 at a time) and particles per row strip --
 trades LDS staging overhead (bigger tiles stage fewer halo cells per particle)
 against parallelism, LDS occupancy and lane utilisation.  The
 best shape depends on the workload (density, replica sizes, GPU fill), so it is
-measured: each candidate runs a few real steps on the live state and the
-force kernel's HIP-event time decides.  Results are cached by a workload
-signature in a small JSON file.
+measured: each candidate runs a few real steps on a clone of the engine (same
+configuration, same current state) and the force kernel's HIP-event time
+decides.  The engine itself is not stepped: its state, step counter and random
+streams are as before, only its tiling changes.  Results are cached by a
+workload signature in a small JSON file.
 """
 
 import json
@@ -24,8 +26,9 @@ def signature(engine):
 
 
 def autotune(engine, candidates=CANDIDATES, steps=4, cache=None):
-    """Pick (tile_rows, tile_particles) for `engine` by timing; leaves the
-    engine set to the best shape.  Returns (best, {shape: ms})."""
+    """Pick (tile_rows, tile_particles) for `engine` by timing steps of a
+    clone of it; leaves the engine set to the best shape and otherwise
+    unchanged.  Returns (best, {shape: ms})."""
     key = signature(engine)
     table = {}
     if cache and os.path.exists(cache):
@@ -36,10 +39,14 @@ def autotune(engine, candidates=CANDIDATES, steps=4, cache=None):
             engine.set_tiling(*best)
             return best, {tuple(map(int, k.split('x'))): v for k, v in table[key]['ms'].items()}
     times = {}
-    for rows, per in candidates:
-        engine.set_tiling(rows, per)
-        engine.time_kernels(1)                      # first launch of this variant
-        times[(rows, per)] = engine.time_kernels(steps)['force'][0]
+    scratch = engine.clone()
+    try:
+        for rows, per in candidates:
+            scratch.set_tiling(rows, per)
+            scratch.time_kernels(1)                 # first launch of this variant
+            times[(rows, per)] = scratch.time_kernels(steps)['force'][0]
+    finally:
+        scratch.close()
     best = min(times, key=times.get)
     engine.set_tiling(*best)
     if cache:

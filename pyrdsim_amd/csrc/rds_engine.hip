@@ -117,8 +117,10 @@ struct DRep {
 
 struct DStatus {
   unsigned long long err_key;  // (step << 32) | (pid*2 + axis), min over errors
-  double err_val;
+  double err_val;              // the coordinate of the particle err_key names
   int bounds, stream;
+  int err_lock;                // guards err_key / err_val together
+  int err_lost;                // a failing lane could not take the lock
   unsigned long long events;
   int integrity;               // bit 0: scatter index out of range, bit 1: histogram total
   int rebuild;                 // list mode: set when a particle moved more than skin/2
@@ -295,6 +297,30 @@ __device__ __forceinline__ double2 rd_pos(const double2* __restrict__ p, int s, 
   if (!cpos) return p[s];
   const float4 v = reinterpret_cast<const float4*>(p)[s];
   return make_double2((double)v.x + (double)v.z, (double)v.y + (double)v.w);
+}
+
+// Records a coordinate that left the box: the smallest key (step, pid, axis)
+// over all failures and the coordinate of the particle it names, written
+// together under a lock (failures are rare, so the lock costs nothing on the
+// normal path).  The lane that takes the lock does its update in the same
+// loop iteration, so lanes of one wave cannot wait on each other; the spin is
+// bounded, and a lane that never gets the lock sets err_lost instead.
+__device__ __noinline__ void record_bounds(DStatus* st, unsigned long long key, double val) {
+  atomicExch(&st->bounds, 1);
+  if (key >= atomicAdd(&st->err_key, 0ull)) return;
+  for (int spin = 0; spin < (1 << 20); ++spin) {
+    if (atomicCAS(&st->err_lock, 0, 1) == 0) {
+      if (key < atomicAdd(&st->err_key, 0ull)) {
+        atomicExch(&st->err_key, key);
+        atomicExch(reinterpret_cast<unsigned long long*>(&st->err_val),
+                   (unsigned long long)__double_as_longlong(val));
+      }
+      __threadfence();
+      atomicExch(&st->err_lock, 0);
+      return;
+    }
+  }
+  atomicExch(&st->err_lost, 1);
 }
 
 __device__ __forceinline__ int cell_coord(double x, double cl, int l) {
@@ -997,13 +1023,9 @@ void k_force(const StepArgs A, int mode, int pw) {
         }
         u2[k] = u;
         const double uv = df_val(u);
-        if (uv < 0.0 || uv > Rp.L[k] || uv != uv) {
-          unsigned long long key = ((unsigned long long)step_next << 32) |
-                                   (unsigned long long)(unsigned)(p * 2 + k);
-          unsigned long long old = atomicMin(&A.status->err_key, key);
-          if (key < old) A.status->err_val = uv;
-          A.status->bounds = 1;
-        }
+        if (uv < 0.0 || uv > Rp.L[k] || uv != uv)
+          record_bounds(A.status, ((unsigned long long)step_next << 32) |
+                                      (unsigned long long)(unsigned)(p * 2 + k), uv);
         v[k] = uv;
       }
       reinterpret_cast<float4*>(A.upos)[s] = make_float4(u2[0].h, u2[1].h, u2[0].l, u2[1].l);
@@ -1040,13 +1062,9 @@ void k_force(const StepArgs A, int mode, int pw) {
           if (m != 0.0 && ((m < 0.0) != (L < 0.0))) m += L;
           u = m;
         }
-        if (u < 0.0 || u > L || u != u) {
-          unsigned long long key = ((unsigned long long)step_next << 32) |
-                                   (unsigned long long)(unsigned)(p * 2 + k);
-          unsigned long long old = atomicMin(&A.status->err_key, key);
-          if (key < old) A.status->err_val = u;
-          A.status->bounds = 1;
-        }
+        if (u < 0.0 || u > L || u != u)
+          record_bounds(A.status, ((unsigned long long)step_next << 32) |
+                                      (unsigned long long)(unsigned)(p * 2 + k), u);
         v[k] = u;
       }
 
@@ -1933,11 +1951,10 @@ struct rds_ctx {
   bool loaded = false;
   long long step0 = 0;
   long long steps_done = 0;
-  // graph
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  // graphs of graph_len steps, one per starting buffer parity
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   int graph_len = 0;
-  int graph_parity = 0;
   // kernel variant
   int rows = 1;         // cell rows per tile (1, 2 or 4)
   bool fast = false;    // all models: <= 2 types, one LJ term per pair, same even n
@@ -2207,29 +2224,34 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
 }
 
 void free_graph(rds_ctx* c) {
-  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-  if (c->graph) hipGraphDestroy(c->graph);
-  c->graph_exec = nullptr;
-  c->graph = nullptr;
+  for (int p = 0; p < 2; ++p) {
+    if (c->graph_exec[p]) hipGraphExecDestroy(c->graph_exec[p]);
+    if (c->graph[p]) hipGraphDestroy(c->graph[p]);
+    c->graph_exec[p] = nullptr;
+    c->graph[p] = nullptr;
+  }
   c->graph_len = 0;
 }
 
+// Captures graph_steps steps twice, once starting from each buffer parity, so
+// that a run replays graphs whatever the parity its previous steps left.
 int build_graph(rds_ctx* c) {
   free_graph(c);
-  int g = c->graph_steps;
+  const int g = c->graph_steps;
   if (g <= 0) return RDS_OK;
-  if (g % 2) ++g;
   const int par = c->parity;
-  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < g; ++k) enqueue_step(c, nullptr);
-  hipGraph_t graph = nullptr;
-  hipError_t e = hipStreamEndCapture(c->stream, &graph);
-  c->parity = par;  // capture did not execute anything
-  if (e != hipSuccess) return fail(RDS_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-  c->graph = graph;
-  HIP_TRY(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+  for (int p = 0; p < 2; ++p) {
+    c->parity = p;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < g; ++k) enqueue_step(c, nullptr);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    c->parity = par;  // capture executed nothing
+    if (e != hipSuccess) return fail(RDS_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    c->graph[p] = graph;
+    HIP_TRY(hipGraphInstantiate(&c->graph_exec[p], c->graph[p], nullptr, nullptr, 0));
+  }
   c->graph_len = g;
-  c->graph_parity = par;
   return RDS_OK;
 }
 
@@ -2861,8 +2883,9 @@ int rds_run(rds_ctx* c, int64_t n_steps) {
   if (c->rng == RDS_RNG_INJECTED && !c->inj_n) return fail(RDS_E_STATE, "no injected streams set");
   int64_t left = n_steps;
   while (left > 0) {
-    if (c->graph_exec && left >= c->graph_len && c->parity == c->graph_parity) {
-      HIP_TRY(hipGraphLaunch(c->graph_exec, c->stream));
+    if (c->graph_len > 0 && left >= c->graph_len) {
+      HIP_TRY(hipGraphLaunch(c->graph_exec[c->parity], c->stream));
+      c->parity ^= c->graph_len & 1;
       left -= c->graph_len;
       c->steps_done += c->graph_len;
     } else {
@@ -2900,6 +2923,10 @@ int rds_sync(rds_ctx* c, rds_status* out) {
     s.error_value = st.err_val;
   }
   if (out) *out = s;
+  if (st.bounds && st.err_lost && st.err_key == ~0ull)
+    return fail(RDS_E_BOUNDS, "a coordinate left the box (the failing particle was not "
+                              "recorded). Maybe the timestep is too large for the given "
+                              "the interaction parameters?");
   if (st.integrity)
     return fail(RDS_E_STATE, "internal cell-list invariant violated (flags %d)", st.integrity);
   if (st.bounds)

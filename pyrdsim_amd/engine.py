@@ -110,7 +110,15 @@ def make_replica(model_index, n_particles, L_box, periodic, dt, tau_by_tauD, rat
     r.n_particles = int(n_particles)
     for k in range(2):
         r.L[k] = float(L_box[k])
-        r.periodic[k] = 1 if float(periodic[k]) > 0 else 0
+        pk = float(periodic[k])
+        if pk == 0.0 or pk != pk:
+            # the reference wraps positions unless periodic < 0
+            # (pyrdsim.pyx:2274) but takes minimum images only where
+            # periodic > 0 (pyrdsim.pyx:1629): 0 mixes the two
+            raise RuntimeError(
+                'periodic[%d] = %r: use a positive value (periodic) or a negative one '
+                '(reflecting); 0 wraps positions but not distances in the reference' % (k, pk))
+        r.periodic[k] = 1 if pk > 0 else 0
     fp, npf = prefactors(dt, tau_by_tauD)
     if len(fp) > L.MAX_TYPES:
         raise RuntimeError('at most %d particle species' % L.MAX_TYPES)
@@ -141,6 +149,13 @@ class Engine:
                  reaction_prob='linear', bonds=None, has_molecules=False,
                  tile_particles=0, graph_steps=0, tile_rows=1, device='cuda',
                  skin=0.0, list_capacity=0, angles=None, angle_threshold=0.1):
+        self._init = (list(models), list(replicas),
+                      dict(precision=precision, rng=rng, reaction_prob=reaction_prob,
+                           bonds=bonds, has_molecules=has_molecules,
+                           tile_particles=tile_particles, graph_steps=graph_steps,
+                           tile_rows=tile_rows, device=device, skin=skin,
+                           list_capacity=list_capacity, angles=angles,
+                           angle_threshold=angle_threshold))
         self.lib = L.load()
         if not torch.cuda.is_available():
             raise RuntimeError('the engine needs a ROCm GPU; none is visible')
@@ -203,6 +218,20 @@ class Engine:
         self._inj = None
         self.step = 0
 
+    def clone(self, rng='philox'):
+        """A new context with this one's configuration, tiling and current
+        state (positions, types, molecules, step counter), for work that must
+        not touch this one -- the autotuner times kernels on a clone.  The
+        clone draws Philox numbers by default: injected streams stay here."""
+        models, reps, kw = self._init
+        kw = dict(kw, rng=rng)
+        rows, per, _ = self.tiling()
+        kw.update(tile_rows=rows, tile_particles=per)
+        e = Engine(models, reps, **kw)
+        x, t = self.state()
+        e.set_state(x, t, getattr(self, '_molecules', None), step0=self.step)
+        return e
+
     def close(self):
         if getattr(self, 'ctx', None):
             self.lib.rds_ctx_destroy(self.ctx)
@@ -234,7 +263,8 @@ class Engine:
         td = self._dev(np.asarray(types, dtype=np.int32).reshape(-1), torch.int32)
         md = None
         if self.has_molecules:
-            md = self._dev(np.asarray(molecules, dtype=np.int32).reshape(-1), torch.int32)
+            self._molecules = np.asarray(molecules, dtype=np.int32).reshape(-1).copy()
+            md = self._dev(self._molecules, torch.int32)
         torch.cuda.synchronize(self.device)
         L.check(self.lib.rds_set_state(self.ctx, xd.data_ptr(), td.data_ptr(),
                                        md.data_ptr() if md is not None else None,
@@ -248,7 +278,8 @@ class Engine:
         a = self._dev(np.asarray(u1, dtype=np.float64), torch.float64)
         b = self._dev(np.asarray(u2, dtype=np.float64), torch.float64) if u2 is not None else None
         S = n.shape[0]
-        if n.numel() != S * self.n_total * 2 or a.numel() != S * self.n_total:
+        if n.numel() != S * self.n_total * 2 or a.numel() != S * self.n_total or \
+                (b is not None and b.numel() != S * self.n_total):
             raise ValueError('injected stream shapes do not match the particle count')
         torch.cuda.synchronize(self.device)
         self._inj = (n, a, b)  # keep alive

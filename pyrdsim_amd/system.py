@@ -144,6 +144,14 @@ class system:
                 "Periodicity definition {0}".format(self.periodic)
                 + " is inconsistent with n_dim = {0}. ".format(self.N_dim)
                 + "Please provide one periodicity value per spatial dimension.")
+        if any(not (float(p) > 0 or float(p) < 0) for p in self.periodic):
+            # the reference wraps positions unless periodic < 0
+            # (pyrdsim.pyx:2274) but takes minimum images only where
+            # periodic > 0 (pyrdsim.pyx:1629): a 0 mixes the two
+            raise RuntimeError(
+                "Periodicity definition {0} holds a 0 (or NaN): use a positive ".format(
+                    self.periodic)
+                + "value for a periodic axis and a negative one for a reflecting axis.")
         self.dt = parameters['dt']
         self.N_steps = parameters['n_steps']
         self.N_out = parameters['n_out']

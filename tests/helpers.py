@@ -107,6 +107,12 @@ def force_scale(s, types=None):
             f = abs(force_law(it, rv)) + abs(rv * fp)
             scale[p] += f
             scale[q] += f
+    for it in s['model']['interactions']:
+        if it['type'] == 'wall':
+            # a wall's natural force unit is its largest force, u0 / (2a); far
+            # from its centre 1 - tanh^2 is a cancellation whose rounding is
+            # of that size times 1e-16, whatever the (vanishing) force itself
+            scale[types == int(it['particle_type'])] += float(it['u0']) / (2.0 * float(it['a']))
     if s['bonds'] is not None:
         # a spring's natural force unit is k * r0 (its force at 100% strain);
         # at rest length its force is rounding noise of that size

@@ -101,6 +101,34 @@ def test_resume_with_append(gpu, workdir):
     assert out['trajectory'].shape == (2, 512, 2)
 
 
+def test_resume_is_continuous(gpu, workdir):
+    """40 steps, then 40 more resumed with append=True from the trajectory
+    file, equal one uninterrupted 80-step run: FP64 positions within 1e-12
+    and identical types.  The resumed run starts from the last frame and the
+    step count the file records (pyrdsim.pyx:1703-1723, 2379, 2410), and the
+    Philox counters continue from that step."""
+    s = workloads.wall(n=1024)
+    s['types'][::3] = 1
+    _write_inputs(workdir, s, n_steps=40, stride=1, n_out=40)
+    system({'seed': 7}).simulate(verbose=False)
+    b = system({'seed': 7, 'n_steps': 80})
+    out_b = b.simulate(verbose=False, append=True)
+    assert b.N_steps_completed == 80
+    c = system({'seed': 7, 'n_steps': 80})
+    out_c = c.simulate(verbose=False)
+    assert c.N_steps_completed == 80
+    d = out_b['x'] - out_c['x']
+    d -= s['L'] * np.round(d / s['L']) * (s['periodic'] > 0)
+    assert np.abs(d).max() < 1e-12, float(np.abs(d).max())
+    assert np.array_equal(out_b['particle_types'], out_c['particle_types'])
+    # the last frames of the two runs' ring buffers are the same step too
+    d = out_b['trajectory'][-1] - out_c['trajectory'][-1]
+    d -= s['L'] * np.round(d / s['L']) * (s['periodic'] > 0)
+    assert np.abs(d).max() < 1e-12
+    # something happened in between: reactions and motion
+    assert (out_c['particle_types'] != s['types']).any()
+
+
 def test_state_file_and_errors(gpu, workdir):
     s = workloads.lj1(n=256)
     _write_inputs(workdir, s, n_steps=10, stride=5, n_out=10)

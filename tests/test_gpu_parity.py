@@ -169,6 +169,32 @@ def test_batched_equals_plain(gpu):
         lo += n
 
 
+def test_autotune_leaves_the_engine_unchanged(gpu):
+    """The autotuner times steps on a clone: the tuned engine keeps its state,
+    step counter and streams, and its next run equals that of an engine built
+    with the chosen tiling that was never tuned."""
+    from pyrdsim_amd.autotune import autotune
+    s = small('wall')
+    s['types'][::3] = 1
+    a = engine_for(s, seed=9, graph_steps=8)
+    x0, t0 = a.state()
+    best, times = autotune(a, candidates=((1, 56), (2, 64), (4, 112)), steps=3)
+    assert set(times) == {(1, 56), (2, 64), (4, 112)}
+    x1, t1 = a.state()
+    assert a.step == 0 and a.sync().steps_done == 0
+    assert np.array_equal(x0, x1) and np.array_equal(t0, t1)
+    assert a.tiling()[:2] == best
+    b = engine_for(s, seed=9, graph_steps=8, tile_rows=best[0], tile_particles=best[1])
+    for e in (a, b):
+        e.run(31)
+    xa, ta = a.state()
+    xb, tb = b.state()
+    d = xa - xb
+    d = d - s['L'] * np.round(d / s['L']) * (s['periodic'] > 0)
+    assert np.abs(d).max() < 1e-12
+    assert np.array_equal(ta, tb)
+
+
 def test_graph_replay_equals_eager(gpu):
     s = small('wall')
     a = engine_for(s, graph_steps=8, seed=3)

@@ -148,6 +148,14 @@ def test_system_defaults_and_errors(tmp_path):
         system(dict(base, periodic=np.array([1.0])))
     with pytest.raises(RuntimeError, match='not recognized'):
         system(dict(base, interactions=[dict(type='morse', pair=[0, 0], u0=1, rc=1)]))
+    # periodic = 0 wraps positions but not distances in the reference: refused
+    with pytest.raises(RuntimeError, match='holds a 0'):
+        system(dict(base, periodic=np.array([1.0, 0.0])))
+    from pyrdsim_amd.engine import make_replica
+    with pytest.raises(RuntimeError, match='periodic'):
+        make_replica(0, 10, [10.0, 10.0], [0.0, 1.0], 0.1, [1.0], [], 1)
+    r = make_replica(0, 10, [10.0, 10.0], [2.0, -0.5], 0.1, [1.0], [], 1)
+    assert list(r.periodic) == [1, 0]
 
 
 def test_bond_and_angle_lists_of_molecules(tmp_path):
