@@ -260,6 +260,21 @@ int rds_type_counts(rds_ctx* ctx, int64_t* counts);
  * type.  Waits for the stream (one host check per propagation pass). */
 int rds_cluster_labels(rds_ctx* ctx, double r_link, uint32_t type_mask, int32_t* labels);
 
+/* Largest particle count (n_total) of rds_cluster_labels_static. */
+#define RDS_STATIC_LABELS_MAX 8388608
+
+/* rds_cluster_labels without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc, set-up code around graph capture).
+ * Same result as rds_cluster_labels; the label array lives in a file-scope
+ * device array of RDS_STATIC_LABELS_MAX int32 entries (32 MiB per GPU, held
+ * until the process ends) instead of memory allocated on first use, and
+ * contexts with more particles get RDS_E_ARG.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other host threads, other
+ * contexts) race and must be serialised by the caller. */
+int rds_cluster_labels_static(rds_ctx* ctx, double r_link, uint32_t type_mask,
+                              int32_t* labels);
+
 /* Times n_steps eager steps with HIP events around every launch on the
  * context's stream; avg_ms[k] for k = 0 force+integrate, 1 scan, 2 scatter,
  * 3 reaction, 4 bonds, 5 list mode's binning + list build (the conditional

@@ -14,6 +14,7 @@ MAX_TYPES = 8
 MAX_TERMS = 32
 MAX_WALLS = 8
 MAX_REACTIONS = 8
+STATIC_LABELS_MAX = 8388608   # RDS_STATIC_LABELS_MAX
 ABI_VERSION = 3
 
 OK, E_ARG, E_HIP, E_BOUNDS, E_STATE, E_STREAM, E_ANGLE = 0, -1, -2, -3, -4, -5, -6
@@ -113,6 +114,7 @@ SIGNATURES = {
     'rds_pair_histogram': (C.c_int, [_VP, C.c_double, C.c_double, C.c_int32, C.c_uint32, _VP]),
     'rds_type_counts': (C.c_int, [_VP, _VP]),
     'rds_cluster_labels': (C.c_int, [_VP, C.c_double, C.c_uint32, _VP]),
+    'rds_cluster_labels_static': (C.c_int, [_VP, C.c_double, C.c_uint32, _VP]),
     'rds_time_kernels': (C.c_int, [_VP, C.c_int64, _VP, _VP]),
     'rds_debug_fill_lds': (C.c_int, [_VP, C.c_uint32]),
     'rds_set_tiling': (C.c_int, [_VP, C.c_int32, C.c_int32]),

@@ -36,6 +36,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1844,6 +1845,8 @@ __global__ void k_cl_jump(int n_total, int* label, const int* __restrict__ slot_
   if (l2 >= 0 && l2 < l) atomicMin(&label[s], l2);
 }
 
+__global__ void k_clear_int(int* p) { *p = 0; }
+
 __global__ void k_cl_out(int n_total, const int* __restrict__ label, const int* __restrict__ pid,
                          int* __restrict__ out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1884,6 +1887,12 @@ __global__ void k_load(int n_total, const double2* __restrict__ x, const int* __
 }
 
 }  // namespace
+
+// Label array of rds_cluster_labels_static, one copy per GPU, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel: the name
+// is a host-side stand-in, and each device's address comes from
+// hipGetSymbolAddress (scratch_ptr below).
+static __device__ __align__(256) int scratch[RDS_STATIC_LABELS_MAX];
 
 // ---------------------------------------------------------------------------
 // host side
@@ -1935,6 +1944,8 @@ struct rds_ctx {
   double2* fbond = nullptr;
   int* slot_of = nullptr;
   int* clabel = nullptr;  // cluster labels (allocated on first use)
+  int* static_labels = nullptr;  // this device's copy of `scratch`
+  int* h_changed = nullptr;      // pinned host word: a label changed in the last pass
   // angles
   int* aoff = nullptr;
   int2* aent = nullptr;
@@ -1977,6 +1988,29 @@ struct rds_ctx {
 };
 
 namespace {
+
+// Device address of `scratch` on device `dev`, resolved once per device with
+// that device current.
+std::mutex g_scratch_mu;
+int* g_scratch_dev[64] = {};
+
+int scratch_ptr(int dev, int** out) {
+  if (dev < 0 || dev >= 64) return fail(RDS_E_ARG, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != dev) HIP_TRY(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIP_TRY(hipSetDevice(cur));
+    if (e != hipSuccess)
+      return fail(RDS_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<int*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return RDS_OK;
+}
 
 StepArgs make_args(rds_ctx* c, const Buf& cur) {
   StepArgs a;
@@ -2472,6 +2506,7 @@ int rds_ctx_destroy(rds_ctx* c) {
   ptrs.erase(std::unique(ptrs.begin(), ptrs.end()), ptrs.end());
   for (void* p : ptrs)
     if (p) hipFree(p);
+  if (c->h_changed) hipHostFree(c->h_changed);
   if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
   delete c;
   return RDS_OK;
@@ -2649,6 +2684,12 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   if (!build_tiles(c, c->rows, (int)target)) return bail(RDS_E_ARG);
 
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  if ((rc = scratch_ptr(c->device, &c->static_labels)) != RDS_OK) return bail(rc);
+  {
+    const hipError_t e = hipHostMalloc((void**)&c->h_changed, sizeof(int), hipHostMallocDefault);
+    if (e != hipSuccess)
+      return bail(fail(RDS_E_HIP, "hipHostMalloc: %s", hipGetErrorString(e)));
+  }
   const size_t N = (size_t)c->n_total, C = (size_t)c->n_cells;
 #define A_(p, n)                                          \
   do {                                                    \
@@ -3043,35 +3084,53 @@ int rds_type_counts(rds_ctx* c, int64_t* counts) {
   return RDS_OK;
 }
 
+// Cluster labels into `lab` (n_total entries of device memory): min-label
+// propagation passes until no label changes, one host check per pass.
+static int cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* labels,
+                          int* lab) {
+  Buf& cur = c->buf[c->parity];
+  const int N = (int)c->n_total;
+  const unsigned mask = type_mask ? type_mask : 0xffffffffu;
+  hipLaunchKernelGGL(k_cl_init, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, cur.type,
+                     cur.pid, mask, lab, c->slot_of);
+  // every pass that changes a label lowers one; N + 1 passes always suffice
+  for (int pass = 0;; ++pass) {
+    if (pass > N) return fail(RDS_E_STATE, "cluster labels did not converge");
+    hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(1), 0, c->stream, &c->d_status->cl_changed);
+    hipLaunchKernelGGL(k_cl_prop, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
+                       c->d_rep, c->d_sblocks, c->cstart, cur.pos, c->cpos, r_link * r_link,
+                       r_link + c->skin, lab, c->slot_of, c->d_status);
+    hipLaunchKernelGGL(k_cl_jump, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, lab,
+                       c->slot_of);
+    HIP_TRY(hipMemcpyAsync(c->h_changed, &c->d_status->cl_changed, sizeof(int),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!*c->h_changed) break;
+  }
+  hipLaunchKernelGGL(k_cl_out, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, lab, cur.pid,
+                     labels);
+  HIP_TRY(hipGetLastError());
+  return RDS_OK;
+}
+
 int rds_cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* labels) {
   if (!c || !labels) return fail(RDS_E_ARG, "null argument");
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   if (int e = check_radius(c, r_link, "r_link")) return e;
   if (!c->clabel && dalloc(&c->clabel, c->n_total) != RDS_OK) return RDS_E_HIP;
-  Buf& cur = c->buf[c->parity];
-  const int N = (int)c->n_total;
-  const unsigned mask = type_mask ? type_mask : 0xffffffffu;
-  hipLaunchKernelGGL(k_cl_init, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, cur.type,
-                     cur.pid, mask, c->clabel, c->slot_of);
-  // every pass that changes a label lowers one; N + 1 passes always suffice
-  for (int pass = 0;; ++pass) {
-    if (pass > N) return fail(RDS_E_STATE, "cluster labels did not converge");
-    HIP_TRY(hipMemsetAsync(&c->d_status->cl_changed, 0, sizeof(int), c->stream));
-    hipLaunchKernelGGL(k_cl_prop, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
-                       c->d_rep, c->d_sblocks, c->cstart, cur.pos, c->cpos, r_link * r_link,
-                       r_link + c->skin, c->clabel, c->slot_of, c->d_status);
-    hipLaunchKernelGGL(k_cl_jump, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->clabel,
-                       c->slot_of);
-    int changed = 0;
-    HIP_TRY(hipMemcpyAsync(&changed, &c->d_status->cl_changed, sizeof(int),
-                           hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (!changed) break;
-  }
-  hipLaunchKernelGGL(k_cl_out, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->clabel,
-                     cur.pid, labels);
-  HIP_TRY(hipGetLastError());
-  return RDS_OK;
+  return cluster_labels(c, r_link, type_mask, labels, c->clabel);
+}
+
+int rds_cluster_labels_static(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* labels) {
+  if (!c || !labels) return fail(RDS_E_ARG, "null argument");
+  if (c->n_total > RDS_STATIC_LABELS_MAX)
+    return fail(RDS_E_ARG,
+                "rds_cluster_labels_static handles at most %d particles (context has %lld); "
+                "use rds_cluster_labels",
+                RDS_STATIC_LABELS_MAX, (long long)c->n_total);
+  if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
+  if (int e = check_radius(c, r_link, "r_link")) return e;
+  return cluster_labels(c, r_link, type_mask, labels, c->static_labels);
 }
 
 int rds_time_kernels(rds_ctx* c, int64_t n_steps, double* avg_ms, int64_t* counts) {
