@@ -17,11 +17,15 @@
 //              loops over its 3x3 stencil in LDS; the force is consumed at once
 //              by the Euler-Maruyama update, the boundary conditions and the
 //              binning of the new position (atomic per-cell histogram + rank)
-//   k_scan     one workgroup per replica: LDS prefix scan of the histogram into
-//              cell starts, histogram cleared for the next step, step counter
-//   k_scatter  counting-sort reorder of the new state into the other buffer
-//   k_react    (models with reactions) neighbour-dependent rates on the same
-//              staged tiles, decision with u1/u2, new type
+//   k_cellsum, k_cellscan
+//              cell starts from the histogram in chunks of <= 2048 cells (a
+//              replica spreads over many workgroups); histogram cleared for
+//              the next step, step counter advanced
+//   k_scatter  counting-sort reorder of the new state into the other buffer;
+//              with reactions it also decides the particles whose rates need
+//              no neighbours and lists the others
+//   k_react_walk (models with catalysed / inhibited reactions) the listed
+//              particles count their neighbours within r0 and decide
 // Positions, the minimum-image displacement and the integrator are FP64; the
 // pair-force arithmetic is FP32 (RDS_MIXED) or FP64 (RDS_FP64).
 //
@@ -94,7 +98,8 @@ struct DRx {
 struct DModel {
   int nt, nterms, nwalls, nrx;
   int nbr_rx;      // any catalysed / inhibited reaction
-  int pad0, pad1, pad2;
+  unsigned walk_types;  // bit t: type t is the reactant of a catalysed / inhibited reaction
+  int pad1, pad2;
   double rc2max;   // largest pair cutoff squared
   int pfirst[MAXT * MAXT];
   int pcount[MAXT * MAXT];
@@ -105,6 +110,7 @@ struct DModel {
 
 struct DRep {
   double L[2], cl[2];
+  double icl[2];          // 1 / cl, rounded
   int lx, ly, cbase, pbase;
   int n, model, px, py;   // periodic axes (boundary conditions)
   unsigned seed_lo, seed_hi;
@@ -185,6 +191,10 @@ struct StepArgs {
   int cpos;             // positions stored as float pairs (compensated FP32)
   const int* boff;      // bond CSR by pid (null without bonds)
   const int* bpart;
+  // cell mode: slots whose reaction rates need a neighbour count, listed by
+  // slot block (sblocks): block b's in wl[256 b ..), their number in wl_n[b]
+  int* wl;
+  int* wl_n;
 };
 
 // ---------------------------------------------------------------------------
@@ -194,8 +204,11 @@ struct StepArgs {
 __device__ __forceinline__ uint4 philox4x32_10(uint4 c, unsigned k0, unsigned k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    unsigned hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-    unsigned hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    // each 32 x 32 -> 64-bit product as one multiply (both halves at once)
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c.x;
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c.z;
+    const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+    const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
     c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
     k0 += 0x9E3779B9u;
     k1 += 0xBB67AE85u;
@@ -329,6 +342,21 @@ __device__ __forceinline__ int cell_coord(double x, double cl, int l) {
   if (!(q >= 0.0)) return 0;
   if (q >= (double)l) return l - 1;
   return (int)q;
+}
+
+// The same cell coordinate from a multiplication by inv = 1/cl (rounded):
+// x*inv is within a few ulp of x/cl, so its floor is floor(x/cl) except
+// where the quotient lies within that of an integer; those rare cases take
+// the division.
+__device__ __forceinline__ int cell_coord_fast(double x, double cl, double inv, int l) {
+  const double t = x * inv;
+  const double q = floor(t);
+  if (t - q > 1e-9 && t - q < 1.0 - 1e-9) {
+    if (!(q >= 0.0)) return 0;
+    if (q >= (double)l) return l - 1;
+    return (int)q;
+  }
+  return cell_coord(x, cl, l);
 }
 
 // Block -> tile map that gives each XCD a contiguous run of tiles (workgroups
@@ -1080,8 +1108,8 @@ void k_force(const StepArgs A, int mode, int pw) {
       if (Rp.py && fabs(ey) > 0.5 * Rp.L[1]) ey -= copysign(Rp.L[1], ey);
       if (ex * ex + ey * ey > A.hskin2) A.status->rebuild = 1;
     } else {
-      const int cx = cell_coord(v[0], Rp.cl[0], Rp.lx);
-      const int cyy = cell_coord(v[1], Rp.cl[1], Rp.ly);
+      const int cx = cell_coord_fast(v[0], Rp.cl[0], Rp.icl[0], Rp.lx);
+      const int cyy = cell_coord_fast(v[1], Rp.cl[1], Rp.icl[1], Rp.ly);
       const int c = Rp.cbase + cyy * Rp.lx + cx;
       A.ucell[s] = c;
       // ranks in the new cells, one atomic per run of lanes with the same new
@@ -1287,19 +1315,28 @@ __device__ __forceinline__ double rx_rate(const DRx& rx, double rate, int ti, in
   return acc;
 }
 
+// Neighbour counts of up to RDS_MAX_REACTIONS reactions packed 16 bits
+// each: reactions 0-3 in lo, 4-7 in hi.  A packed word keeps loops over the
+// reactions rolled without an indexed array in scratch memory.
+__device__ __forceinline__ int cnt_of(unsigned long long lo, unsigned long long hi, int q) {
+  return (int)(((q < 4 ? lo : hi) >> (16 * (q & 3))) & 0xffffull);
+}
+__device__ __forceinline__ void cnt_add(unsigned long long& lo, unsigned long long& hi, int q) {
+  if (q < 4) lo += 1ull << (16 * q);
+  else hi += 1ull << (16 * (q - 4));
+}
+
 // Rates, draws and choice of one particle (pyrdsim.pyx:2103-2222): the new
-// type of particle p (type ti, neighbour counts cnt per reaction) in step
-// `step`.
+// type of particle p (type ti, neighbour counts per reaction packed in
+// c_lo / c_hi) in step `step`.
 __device__ __forceinline__ int react_decide(const StepArgs& A, const DRep& Rp, const DModel& M,
-                                            int ti, int p, long long step, const int* cnt) {
+                                            int ti, int p, long long step,
+                                            unsigned long long c_lo, unsigned long long c_hi) {
   const int nrx = M.nrx;
   // total rate in reaction order (pyrdsim.pyx:2103-2206)
   double total_rate = 0.0;
-#pragma unroll
-  for (int q = 0; q < MAXRX; ++q) {
-    if (q >= nrx) break;
-    rx_rate(M.rx[q], Rp.rate[q], ti, cnt[q], total_rate);
-  }
+#pragma unroll 1
+  for (int q = 0; q < nrx; ++q) rx_rate(M.rx[q], Rp.rate[q], ti, cnt_of(c_lo, c_hi, q), total_rate);
   if (!(total_rate > 0.0)) return ti;
   double u1, u2;
   if (A.rng == RDS_RNG_INJECTED) {
@@ -1322,113 +1359,250 @@ __device__ __forceinline__ int react_decide(const StepArgs& A, const DRep& Rp, c
   // (pyrdsim.pyx:2215-2222); the rates are recomputed bit-identically
   double sacc = 0.0, dummy = 0.0;
   int chosen = -1;
-#pragma unroll
-  for (int q = 0; q < MAXRX; ++q) {
-    if (q >= nrx || chosen >= 0) break;
-    sacc += rx_rate(M.rx[q], Rp.rate[q], ti, cnt[q], dummy) / total_rate;
-    if (sacc >= u2) chosen = q;
+#pragma unroll 1
+  for (int q = 0; q < nrx; ++q) {
+    sacc += rx_rate(M.rx[q], Rp.rate[q], ti, cnt_of(c_lo, c_hi, q), dummy) / total_rate;
+    if (sacc >= u2) {
+      chosen = q;
+      break;
+    }
   }
   if (chosen < 0) return ti;
   atomicAdd(&A.status->events, 1ull);
   return M.rx[chosen].product;
 }
 
+// New type of slot s of replica Rp (pre-reaction types in A.type): its
+// neighbour counts within each reaction's r0 when its type is the reactant of
+// a catalysed or inhibited reaction, then the decision.
+template <bool MOL>
+__device__ int react_slot(const StepArgs& A, const DRep& Rp, int s) {
+  const DModel& M = A.models[Rp.model];
+  const int nrx = M.nrx;
+  const int ti = A.type[s];
+  if (nrx == 0) return ti;
+  const int p = A.pid[s];
+  const long long step = *A.step;  // already advanced: the reference's l+1
+  bool need = false;
+  unsigned nbr_types = 0u;  // neighbour types that any of ti's reactions counts
+  for (int q = 0; q < nrx; ++q) {
+    const DRx& rx = M.rx[q];
+    if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
+    need = true;
+    const int a = rx.kind == RDS_RX_CATALYZED ? rx.catalyst : rx.inh;
+    const int b = rx.kind == RDS_RX_CATALYZED ? -1 : rx.inh2;
+    if (a >= 0 && a < 32) nbr_types |= 1u << a;
+    if (b >= 0 && b < 32) nbr_types |= 1u << b;
+  }
+  unsigned long long c_lo = 0ull, c_hi = 0ull;
+  if (need) {
+    const double2 xi = rd_pos(A.pos, s, A.cpos);
+    const int mi = MOL ? A.mol[s] : 0;
+    // the cell the slot is sorted into (list mode: of the last rebuild; cells
+    // are then at least range + skin wide, so the 3x3 stencil still holds
+    // every particle within the range)
+    int cx, cy;
+    if (A.scell) {
+      const int c = A.scell[s] - Rp.cbase;
+      cy = c / Rp.lx;
+      cx = c - cy * Rp.lx;
+    } else {
+      cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
+      cy = cell_coord(xi.y, Rp.cl[1], Rp.ly);
+    }
+    for (int oy = -1; oy <= 1; ++oy) {
+      int y = cy + oy;
+      if (y < 0 || y >= Rp.ly) {
+        if (!Rp.sy) continue;
+        y = (y < 0) ? y + Rp.ly : y - Rp.ly;
+      }
+      for (int ox = -1; ox <= 1; ++ox) {
+        int x = cx + ox;
+        if (x < 0 || x >= Rp.lx) {
+          if (!Rp.sx) continue;
+          x = (x < 0) ? x + Rp.lx : x - Rp.lx;
+        }
+        const int c = Rp.cbase + y * Rp.lx + x;
+        const int j1 = A.cstart[c + 1];
+        for (int j = A.cstart[c]; j < j1; ++j) {
+          if (j == s) continue;
+          // only neighbours of a counted type need a distance
+          const int tj = A.type[j];
+          if (tj < 0 || tj >= 32 || !((nbr_types >> tj) & 1u)) continue;
+          // the reference's neighbour list holds other molecules' particles
+          // and the bonded partners (pyrdsim.pyx:1513, 1523)
+          if (MOL && A.mol[j] == mi) {
+            bool bonded = false;
+            if (A.boff) {
+              const int pj = A.pid[j];
+              for (int b = A.boff[p]; b < A.boff[p + 1]; ++b) bonded |= A.bpart[b] == pj;
+            }
+            if (!bonded) continue;
+          }
+          const double2 xj = rd_pos(A.pos, j, A.cpos);
+          double dx = xj.x - xi.x, dy = xj.y - xi.y;
+          if (Rp.px) dx = min_image(dx, Rp.L[0]);
+          if (Rp.py) dy = min_image(dy, Rp.L[1]);
+          const double d2 = dx * dx + dy * dy;
+#pragma unroll 1
+          for (int q = 0; q < nrx; ++q) {
+            const DRx& rx = M.rx[q];
+            if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
+            const bool match = (rx.kind == RDS_RX_CATALYZED) ? (tj == rx.catalyst)
+                                                             : (tj == rx.inh || tj == rx.inh2);
+            // d < r0 with d = sqrt(d2) as the reference computes it; away
+            // from r0^2 (relative 1e-14) d2 alone decides it
+            const bool within = d2 < rx.r0sq_lo ||
+                                (d2 <= rx.r0sq_hi && sqrt(d2) < rx.r0);
+            if (match && within) cnt_add(c_lo, c_hi, q);
+          }
+        }
+      }
+    }
+  }
+  return react_decide(A, Rp, M, ti, p, step, c_lo, c_hi);
+}
+
+// List mode: every slot, one thread each, in per-replica slot blocks.
 template <bool MOL>
 __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
   const int4 B = A.sblocks[blockIdx.x];  // (replica, first slot, end slot)
   const int s = B.y + (int)threadIdx.x;
   if (s >= B.z) return;
+  A.type_out[s] = react_slot<MOL>(A, A.rep[B.x], s);
+}
+
+// Cell mode: the slots k_scatter_react listed (reactants of a catalysed or
+// inhibited reaction), one wave per slot block, one lane per listed slot.
+// The block's replica is wave-uniform (its tables come through scalar
+// loads), and the parameters of the slot's neighbour-dependent reactions
+// are taken into registers before its candidate loop; each stencil row is
+// at most two contiguous slot ranges, walked four candidates at a time.
+template <bool MOL>
+__global__ __launch_bounds__(256) void k_react_walk(const StepArgs A, int n_blocks) {
+  const int sb = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (sb >= n_blocks) return;
+  const int n = A.wl_n[sb];
+  if (n == 0) return;
+  const int4 B = A.sblocks[sb];
   const DRep& Rp = A.rep[B.x];
   const DModel& M = A.models[Rp.model];
   const int nrx = M.nrx;
-  const int ti = A.type[s];
-  int tnew = ti;
-  if (nrx > 0) {
+  const long long step = *A.step;
+  const int lx = Rp.lx, ly = Rp.ly, cbase = Rp.cbase, sx = Rp.sx, sy = Rp.sy;
+  const int px = Rp.px, py = Rp.py;
+  const double L0 = Rp.L[0], L1 = Rp.L[1], cl0 = Rp.cl[0], cl1 = Rp.cl[1];
+  for (int i = threadIdx.x & 63; i < n; i += 64) {
+    const int s = A.wl[sb * 256 + i];
+    const int ti = A.type[s];
     const int p = A.pid[s];
-    const long long step = *A.step;  // already advanced: the reference's l+1
-    bool need = false;
-    unsigned nbr_types = 0u;  // neighbour types that any of ti's reactions counts
+    const double2 xi = rd_pos(A.pos, s, A.cpos);
+    const int mi = MOL ? A.mol[s] : 0;
+    // the slot's neighbour-dependent reactions: the first two in registers
+    // (match mask, r0 band), any further ones through the table
+    unsigned nbr_types = 0u, mask_a = 0u, mask_b = 0u;
+    int qa = -1, qb = -1;
+    bool more = false;
+#pragma unroll 1
     for (int q = 0; q < nrx; ++q) {
       const DRx& rx = M.rx[q];
       if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
-      need = true;
+      unsigned mk = 0u;
       const int a = rx.kind == RDS_RX_CATALYZED ? rx.catalyst : rx.inh;
       const int b = rx.kind == RDS_RX_CATALYZED ? -1 : rx.inh2;
-      if (a >= 0 && a < 32) nbr_types |= 1u << a;
-      if (b >= 0 && b < 32) nbr_types |= 1u << b;
+      if (a >= 0 && a < 32) mk |= 1u << a;
+      if (b >= 0 && b < 32) mk |= 1u << b;
+      nbr_types |= mk;
+      if (qa < 0) { qa = q; mask_a = mk; }
+      else if (qb < 0) { qb = q; mask_b = mk; }
+      else more = true;
     }
-    int cnt[MAXRX];
-#pragma unroll
-    for (int q = 0; q < MAXRX; ++q) cnt[q] = 0;
-    if (need) {
-      const double2 xi = rd_pos(A.pos, s, A.cpos);
-      const int mi = MOL ? A.mol[s] : 0;
-      // the cell the slot is sorted into (list mode: of the last rebuild; cells
-      // are then at least range + skin wide, so the 3x3 stencil still holds
-      // every particle within the range)
-      int cx, cy;
-      if (A.scell) {
-        const int c = A.scell[s] - Rp.cbase;
-        cy = c / Rp.lx;
-        cx = c - cy * Rp.lx;
-      } else {
-        cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
-        cy = cell_coord(xi.y, Rp.cl[1], Rp.ly);
+    double lo_a = 0.0, hi_a = 0.0, r0_a = 0.0, lo_b = 0.0, hi_b = 0.0, r0_b = 0.0;
+    if (qa >= 0) { lo_a = M.rx[qa].r0sq_lo; hi_a = M.rx[qa].r0sq_hi; r0_a = M.rx[qa].r0; }
+    if (qb >= 0) { lo_b = M.rx[qb].r0sq_lo; hi_b = M.rx[qb].r0sq_hi; r0_b = M.rx[qb].r0; }
+    unsigned long long c_lo = 0ull, c_hi = 0ull;  // packed counts (cnt_of)
+    const int cx = cell_coord(xi.x, cl0, lx);
+    const int cy = cell_coord(xi.y, cl1, ly);
+#pragma unroll 1
+    for (int oy = -1; oy <= 1; ++oy) {
+      int y = cy + oy;
+      if (y < 0 || y >= ly) {
+        if (!sy) continue;
+        y = (y < 0) ? y + ly : y - ly;
       }
-      for (int oy = -1; oy <= 1; ++oy) {
-        int y = cy + oy;
-        if (y < 0 || y >= Rp.ly) {
-          if (!Rp.sy) continue;
-          y = (y < 0) ? y + Rp.ly : y - Rp.ly;
-        }
-        for (int ox = -1; ox <= 1; ++ox) {
-          int x = cx + ox;
-          if (x < 0 || x >= Rp.lx) {
-            if (!Rp.sx) continue;
-            x = (x < 0) ? x + Rp.lx : x - Rp.lx;
-          }
-          const int c = Rp.cbase + y * Rp.lx + x;
-          const int j1 = A.cstart[c + 1];
-          for (int j = A.cstart[c]; j < j1; ++j) {
-            if (j == s) continue;
-            // only neighbours of a counted type need a distance
-            const int tj = A.type[j];
-            if (tj < 0 || tj >= 32 || !((nbr_types >> tj) & 1u)) continue;
-            // the reference's neighbour list holds other molecules' particles
-            // and the bonded partners (pyrdsim.pyx:1513, 1523)
-            if (MOL && A.mol[j] == mi) {
-              bool bonded = false;
-              if (A.boff) {
-                const int pj = A.pid[j];
-                for (int b = A.boff[p]; b < A.boff[p + 1]; ++b) bonded |= A.bpart[b] == pj;
-              }
-              if (!bonded) continue;
-            }
-            const double2 xj = rd_pos(A.pos, j, A.cpos);
-            double dx = xj.x - xi.x, dy = xj.y - xi.y;
-            if (Rp.px) dx = min_image(dx, Rp.L[0]);
-            if (Rp.py) dy = min_image(dy, Rp.L[1]);
-            const double d2 = dx * dx + dy * dy;
+      // cells cx-1 .. cx+1 of the row as at most two contiguous slot ranges
+      const int rowc = cbase + y * lx;
+      int a0, a1, b0r = 0, b1r = 0;
+      if (!sx) {
+        a0 = A.cstart[rowc + max(cx - 1, 0)];
+        a1 = A.cstart[rowc + min(cx + 1, lx - 1) + 1];
+      } else if (cx == 0) {
+        a0 = A.cstart[rowc + lx - 1];
+        a1 = A.cstart[rowc + lx];
+        b0r = A.cstart[rowc];
+        b1r = A.cstart[rowc + 2];
+      } else if (cx == lx - 1) {
+        a0 = A.cstart[rowc + cx - 1];
+        a1 = A.cstart[rowc + lx];
+        b0r = A.cstart[rowc];
+        b1r = A.cstart[rowc + 1];
+      } else {
+        a0 = A.cstart[rowc + cx - 1];
+        a1 = A.cstart[rowc + cx + 2];
+      }
+      const int na = a1 - a0, nn = na + (b1r - b0r);
+#pragma unroll 1
+      for (int u = 0; u < nn; u += 4) {
+        int jj[4], tt[4];
 #pragma unroll
-            for (int q = 0; q < MAXRX; ++q) {
-              if (q >= nrx) break;
+        for (int v = 0; v < 4; ++v) {
+          const int w = u + v;
+          jj[v] = w < na ? a0 + w : b0r + (w - na);
+          tt[v] = w < nn ? A.type[jj[v]] : -1;
+        }
+#pragma unroll 1
+        for (int v = 0; v < 4; ++v) {
+          const int j = jj[v], tj = tt[v];
+          if (j == s || tj < 0 || tj >= 32 || !((nbr_types >> tj) & 1u)) continue;
+          // the reference's neighbour list holds other molecules' particles
+          // and the bonded partners (pyrdsim.pyx:1513, 1523)
+          if (MOL && A.mol[j] == mi) {
+            bool bonded = false;
+            if (A.boff) {
+              const int pj = A.pid[j];
+              for (int bb = A.boff[p]; bb < A.boff[p + 1]; ++bb) bonded |= A.bpart[bb] == pj;
+            }
+            if (!bonded) continue;
+          }
+          const double2 xj = rd_pos(A.pos, j, A.cpos);
+          double dx = xj.x - xi.x, dy = xj.y - xi.y;
+          if (px) dx = min_image(dx, L0);
+          if (py) dy = min_image(dy, L1);
+          const double d2 = dx * dx + dy * dy;
+          // d < r0 with d = sqrt(d2) as the reference computes it; away from
+          // r0^2 (relative 1e-14) d2 alone decides it
+          if (((mask_a >> tj) & 1u) && (d2 < lo_a || (d2 <= hi_a && sqrt(d2) < r0_a)))
+            cnt_add(c_lo, c_hi, qa);
+          if (((mask_b >> tj) & 1u) && (d2 < lo_b || (d2 <= hi_b && sqrt(d2) < r0_b)))
+            cnt_add(c_lo, c_hi, qb);
+          if (more) {
+            int seen = 0;
+#pragma unroll 1
+            for (int q = 0; q < nrx; ++q) {
               const DRx& rx = M.rx[q];
               if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
+              if (seen++ < 2) continue;  // the first two are counted above
               const bool match = (rx.kind == RDS_RX_CATALYZED) ? (tj == rx.catalyst)
                                                                : (tj == rx.inh || tj == rx.inh2);
-              // d < r0 with d = sqrt(d2) as the reference computes it; away
-              // from r0^2 (relative 1e-14) d2 alone decides it
-              const bool within = d2 < rx.r0sq_lo ||
-                                  (d2 <= rx.r0sq_hi && sqrt(d2) < rx.r0);
-              if (match && within) cnt[q] += 1;
+              if (match && (d2 < rx.r0sq_lo || (d2 <= rx.r0sq_hi && sqrt(d2) < rx.r0)))
+                cnt_add(c_lo, c_hi, q);
             }
           }
         }
       }
     }
-    tnew = react_decide(A, Rp, M, ti, p, step, cnt);
+    A.type_out[s] = react_decide(A, Rp, M, ti, p, step, c_lo, c_hi);
   }
-  A.type_out[s] = tnew;
 }
 
 // ---------------------------------------------------------------------------
@@ -1621,31 +1795,171 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const DRep* __restrict__ re
   if (advance && blockIdx.x == 0 && t == 0) *step = *step + 1;
 }
 
-__global__ void k_scatter(int n_total, const double2* __restrict__ upos,
-                          const int* __restrict__ ucell, const int* __restrict__ urank,
-                          const int* __restrict__ cstart, const int* __restrict__ type,
-                          const int* __restrict__ pid, const int* __restrict__ mol,
-                          double2* __restrict__ npos, int* __restrict__ ntype,
-                          int* __restrict__ npid, int* __restrict__ nmol,
-                          int* __restrict__ slot_of, int n_cells, DStatus* status) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+// Counting-sort reorder of the new state into the other buffer, one thread
+// per slot.
+__global__ __launch_bounds__(256) void k_scatter(
+    int n_total, const double2* __restrict__ upos, const int* __restrict__ ucell,
+    const int* __restrict__ urank, const int* __restrict__ cstart, const int* __restrict__ type,
+    const int* __restrict__ pid, const int* __restrict__ mol, double2* __restrict__ npos,
+    int* __restrict__ ntype, int* __restrict__ npid, int* __restrict__ nmol,
+    int* __restrict__ slot_of, int n_cells, DStatus* status) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_total) return;
   const int c = ucell[s];
   if (c < 0 || c >= n_cells) {
     atomicOr(&status->integrity, 1);
     return;
   }
-  int d = cstart[c] + urank[s];
+  const int d = cstart[c] + urank[s];
   if (d < 0 || d >= n_total) {
     atomicOr(&status->integrity, 1);
     return;
   }
-  int p = pid[s];
+  const int p = pid[s];
   npos[d] = upos[s];
   ntype[d] = type[s];
   npid[d] = p;
   if (nmol) nmol[d] = mol[s];
   if (slot_of) slot_of[p] = d;
+}
+
+// The same reorder for models with reactions (cell mode, after the step
+// counter advanced), over the per-replica slot blocks (A.sblocks: one
+// replica per workgroup, so its tables are read with scalar loads): the
+// pre-reaction type goes to ntype0, and the reaction step is decided here
+// for every particle whose rates need no neighbours (pyrdsim.pyx:2103-2222);
+// the others are listed for k_react_walk in the block's own segment of the
+// worklist, compacted within the block.
+__global__ __launch_bounds__(256) void k_scatter_react(
+    const double2* __restrict__ upos, const int* __restrict__ ucell,
+    const int* __restrict__ urank, const int* __restrict__ cstart, const int* __restrict__ type,
+    const int* __restrict__ pid, const int* __restrict__ mol, double2* __restrict__ npos,
+    int* __restrict__ ntype, int* __restrict__ npid, int* __restrict__ nmol,
+    int* __restrict__ slot_of, int n_cells, int* __restrict__ ntype0, const StepArgs A) {
+  __shared__ int s_wcnt[4];
+  const int4 B = A.sblocks[blockIdx.x];  // (replica, first slot, end slot)
+  const DRep& Rp = A.rep[B.x];
+  const DModel& M = A.models[Rp.model];
+  const unsigned walk_types = M.walk_types;
+  const int nrx = M.nrx;
+  const long long step = *A.step;
+  const int s = B.y + (int)threadIdx.x;
+  bool walker = false;
+  int d = -1;
+  if (s < B.z) {
+    const int c = ucell[s];
+    if (c < 0 || c >= n_cells) {
+      atomicOr(&A.status->integrity, 1);
+    } else {
+      d = cstart[c] + urank[s];
+      if (d < 0 || d >= A.n_total) {
+        atomicOr(&A.status->integrity, 1);
+        d = -1;
+      }
+    }
+    if (d >= 0) {
+      const int p = pid[s];
+      const int t = type[s];
+      npos[d] = upos[s];
+      npid[d] = p;
+      if (nmol) nmol[d] = mol[s];
+      if (slot_of) slot_of[p] = d;
+      ntype0[d] = t;
+      if ((walk_types >> t) & 1u) walker = true;
+      else ntype[d] = nrx > 0 ? react_decide(A, Rp, M, t, p, step, 0ull, 0ull) : t;
+    }
+  }
+  const unsigned long long m = __ballot(walker);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) s_wcnt[wv] = __popcll(m);
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < wv; ++w) base += s_wcnt[w];
+  if (walker) A.wl[blockIdx.x * 256 + base + __popcll(m & ((1ull << lane) - 1ull))] = d;
+  if (threadIdx.x == 0) A.wl_n[blockIdx.x] = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+}
+
+// Cell starts in chunks of at most CHUNK_CELLS cells of one replica
+// (chunk = (replica, first cell within the replica, cells, index of the
+// replica's first chunk)), so that large replicas spread over many
+// workgroups.  k_cellsum: particles per chunk.  k_cellscan: the chunk's base
+// (pbase + the sums of the replica's earlier chunks) plus a workgroup scan of
+// its histogram into cell starts; the histogram is cleared for the next
+// step and the step counter advanced (advance).
+#define CHUNK_CELLS 2048
+#define CHUNK_THREADS 256
+
+__device__ __forceinline__ int block_sum_256(int v, int* s_red) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((t & 63) == 0) s_red[t >> 6] = v;
+  __syncthreads();
+  const int tot = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  __syncthreads();
+  return tot;
+}
+
+__global__ __launch_bounds__(CHUNK_THREADS) void k_cellsum(const int4* __restrict__ chunks,
+                                                           const DRep* __restrict__ rep,
+                                                           const int* __restrict__ count,
+                                                           int* __restrict__ csum) {
+  __shared__ int s_red[4];
+  const int4 ch = chunks[blockIdx.x];
+  const int c0 = rep[ch.x].cbase + ch.y;
+  int v = 0;
+  for (int i = threadIdx.x; i < ch.z; i += CHUNK_THREADS) v += count[c0 + i];
+  const int tot = block_sum_256(v, s_red);
+  if (threadIdx.x == 0) csum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(CHUNK_THREADS) void k_cellscan(
+    const int4* __restrict__ chunks, const DRep* __restrict__ rep, int* __restrict__ count,
+    int* __restrict__ cstart, const int* __restrict__ csum, long long* step, int advance,
+    DStatus* status) {
+  __shared__ int s_red[4];
+  __shared__ int s_wsum[CHUNK_THREADS / 64];
+  const int t = threadIdx.x, b = blockIdx.x;
+  const int4 ch = chunks[b];
+  const DRep& R = rep[ch.x];
+  const int c0 = R.cbase + ch.y;
+  // particles in the replica's chunks before this one
+  int pre = 0;
+  for (int k = ch.w + t; k < b; k += CHUNK_THREADS) pre += csum[k];
+  const int base = R.pbase + block_sum_256(pre, s_red);
+  constexpr int PER = CHUNK_CELLS / CHUNK_THREADS;
+  int v[PER];
+  int tsum = 0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int c = t * PER + i;
+    v[i] = c < ch.z ? count[c0 + c] : 0;
+    tsum += v[i];
+  }
+  int inc = tsum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(inc, d, 64);
+    if ((t & 63) >= d) inc += o;
+  }
+  if ((t & 63) == 63) s_wsum[t >> 6] = inc;
+  __syncthreads();
+  int wbase = 0;
+  for (int w = 0; w < (t >> 6); ++w) wbase += s_wsum[w];
+  int excl = base + wbase + inc - tsum;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int c = t * PER + i;
+    if (c < ch.z) {
+      cstart[c0 + c] = excl;
+      count[c0 + c] = 0;
+    }
+    excl += v[i];
+  }
+  // the replica's last chunk checks the total
+  if (t == CHUNK_THREADS - 1 && ch.y + ch.z == R.lx * R.ly && excl != R.pbase + R.n)
+    atomicOr(&status->integrity, 2);
+  if (advance && b == 0 && t == 0) *step = *step + 1;
 }
 
 // List mode rebuild: counting-sort reorder of the slots into scratch columns
@@ -1934,6 +2248,11 @@ struct rds_ctx {
   int* urank = nullptr;
   int* count = nullptr;
   int* cstart = nullptr;
+  int4* d_chunks = nullptr;  // cell-start chunks (replica, first cell, cells, first chunk)
+  int* d_csum = nullptr;     // particles per chunk
+  int64_t n_chunks = 0;
+  int* wl = nullptr;         // reaction worklist (cell mode), 256 entries per slot block
+  int* wl_n = nullptr;
   long long* d_step = nullptr;
   DStatus* d_status = nullptr;
   // bonds
@@ -2052,6 +2371,8 @@ StepArgs make_args(rds_ctx* c, const Buf& cur) {
   a.cpos = c->cpos;
   a.boff = c->any_bonds ? c->boff : nullptr;
   a.bpart = c->any_bonds ? c->bpart : nullptr;
+  a.wl = c->wl;
+  a.wl_n = c->wl_n;
   return a;
 }
 
@@ -2097,6 +2418,16 @@ inline unsigned nblk_gs(int64_t n) { return std::min(nblk(n, 256), 4096u); }
 
 // kernel ids for timing
 enum { KT_FORCE = 0, KT_SCAN = 1, KT_SCATTER = 2, KT_REACT = 3, KT_BONDS = 4, KT_LIST = 5, KT_N = 6 };
+
+// Cell starts from the histogram (chunked scan); advances the step counter
+// when `advance`.
+void launch_cell_starts(rds_ctx* c, int advance) {
+  hipLaunchKernelGGL(k_cellsum, dim3((unsigned)c->n_chunks), dim3(CHUNK_THREADS), 0, c->stream,
+                     c->d_chunks, c->d_rep, c->count, c->d_csum);
+  hipLaunchKernelGGL(k_cellscan, dim3((unsigned)c->n_chunks), dim3(CHUNK_THREADS), 0, c->stream,
+                     c->d_chunks, c->d_rep, c->count, c->cstart, c->d_csum, c->d_step, advance,
+                     c->d_status);
+}
 
 template <bool MOL>
 void launch_build(rds_ctx* c, const StepArgs& a) {
@@ -2234,24 +2565,34 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
   launch_force(c, a, MODE_STEP);
   mark(KT_FORCE, false);
   mark(KT_SCAN, true);
-  hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep, c->count,
-                     c->cstart, c->d_step, 1, c->d_status, 0);
+  launch_cell_starts(c, 1);
   mark(KT_SCAN, false);
+  // reactions: the scatter decides the particles whose rates need no
+  // neighbours and lists the others for k_react_walk, which reads the
+  // pre-reaction types (type0) of the new buffer
+  StepArgs r = make_args(c, nxt);
+  r.type = nxt.type0;
+  r.type_out = nxt.type;
   mark(KT_SCATTER, true);
-  hipLaunchKernelGGL(k_scatter, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
-                     (int)c->n_total, c->upos, c->ucell, c->urank, c->cstart, cur.type, cur.pid,
-                     cur.mol, nxt.pos, c->any_rx ? nxt.type0 : nxt.type, nxt.pid, nxt.mol,
-                     c->any_bonds ? c->slot_of : nullptr, (int)c->n_cells, c->d_status);
+  if (c->any_rx)
+    hipLaunchKernelGGL(k_scatter_react, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
+                       c->upos, c->ucell, c->urank, c->cstart, cur.type, cur.pid, cur.mol,
+                       nxt.pos, nxt.type, nxt.pid, nxt.mol, c->any_bonds ? c->slot_of : nullptr,
+                       (int)c->n_cells, nxt.type0, r);
+  else
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                       (int)c->n_total, c->upos, c->ucell, c->urank, c->cstart, cur.type,
+                       cur.pid, cur.mol, nxt.pos, nxt.type, nxt.pid, nxt.mol,
+                       c->any_bonds ? c->slot_of : nullptr, (int)c->n_cells, c->d_status);
   mark(KT_SCATTER, false);
   if (c->any_rx) {
-    StepArgs r = make_args(c, nxt);
-    r.type = nxt.type0;
-    r.type_out = nxt.type;
     mark(KT_REACT, true);
+    const int nb = (int)c->n_sblocks;  // one wave per slot block
+    const unsigned g = (unsigned)((nb + 3) / 4);
     if (c->has_mol)
-      hipLaunchKernelGGL((k_react<true>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
+      hipLaunchKernelGGL((k_react_walk<true>), dim3(g), dim3(256), 0, c->stream, r, nb);
     else
-      hipLaunchKernelGGL((k_react<false>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
+      hipLaunchKernelGGL((k_react_walk<false>), dim3(g), dim3(256), 0, c->stream, r, nb);
     mark(KT_REACT, false);
   }
   c->parity ^= 1;
@@ -2389,6 +2730,7 @@ int fill_model(const rds_model& m, DModel& d, double* rcut) {
       return fail(RDS_E_ARG, "reaction %d: reactant/product outside the %d particle types", q, nt);
     if (r.kind == RDS_RX_CATALYZED || r.kind == RDS_RX_INHIBITED) {
       d.nbr_rx = 1;
+      d.walk_types |= 1u << r.reactant;
       rc = std::max(rc, r.r0);
     } else if (r.kind != RDS_RX_LINEAR) {
       return fail(RDS_E_ARG, "unknown reaction kind %d", r.kind);
@@ -2494,6 +2836,7 @@ int rds_ctx_destroy(rds_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   std::vector<void*> ptrs = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->d_sblocks,
                              c->upos, c->ucell, c->urank, c->count, c->cstart, c->d_step,
+                             c->d_chunks, c->d_csum, c->wl, c->wl_n,
                              c->d_status, c->boff, c->bpart, c->br0, c->bu0, c->fbond,
                              c->slot_of, c->clabel, c->aoff, c->aent, c->apart, c->aprm, c->nl4, c->ncnt, c->xref, c->scell, c->spos,
                              c->stype, c->spid, c->smol};
@@ -2639,6 +2982,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
       lcell[k] = l;
       d.L[k] = s.L[k];
       d.cl[k] = s.L[k] / (double)l;
+      d.icl[k] = 1.0 / d.cl[k];
     }
     // a periodic axis with fewer than 3 cells (only possible without any pair
     // range) is treated as one cell with no neighbours: nothing interacts
@@ -2737,6 +3081,19 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   A_(c->urank, N);
   A_(c->count, C);
   A_(c->cstart, C + 1);
+  {
+    std::vector<int4> ch;
+    for (int r = 0; r < c->n_rep; ++r) {
+      const DRep& d = c->h_rep[r];
+      const int cr = d.lx * d.ly, first = (int)ch.size();
+      for (int o = 0; o < cr; o += CHUNK_CELLS)
+        ch.push_back(make_int4(r, o, std::min(CHUNK_CELLS, cr - o), first));
+    }
+    c->n_chunks = (int64_t)ch.size();
+    A_(c->d_chunks, ch.size());
+    A_(c->d_csum, ch.size());
+    HIP_TRY(hipMemcpy(c->d_chunks, ch.data(), sizeof(int4) * ch.size(), hipMemcpyHostToDevice));
+  }
   A_(c->d_step, 1);
   A_(c->d_status, 1);
   A_(c->slot_of, N);
@@ -2756,6 +3113,9 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     HIP_TRY(hipMemcpy(c->d_rep_of, rep_of.data(), sizeof(int) * N, hipMemcpyHostToDevice));
     c->n_sblocks = (int64_t)sb.size();
     if ((rc = dalloc(&c->d_sblocks, sb.size())) != RDS_OK) return bail(rc);
+    // reaction worklist: a segment of 256 entries per slot block
+    if ((rc = dalloc(&c->wl, sb.size() * 256)) != RDS_OK) return bail(rc);
+    if ((rc = dalloc(&c->wl_n, sb.size())) != RDS_OK) return bail(rc);
     HIP_TRY(hipMemcpy(c->d_sblocks, sb.data(), sizeof(int4) * sb.size(), hipMemcpyHostToDevice));
   }
   HIP_TRY(hipMemset(c->count, 0, sizeof(int) * C));
@@ -2871,8 +3231,7 @@ int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32
   } else {
     hipLaunchKernelGGL(k_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep,
                        c->d_rep_of, a.pos, c->ucell, c->urank, c->count, nullptr, c->cpos);
-    hipLaunchKernelGGL(k_scan, dim3(c->n_rep), dim3(SCAN_BLOCK), 0, c->stream, c->d_rep,
-                       c->count, c->cstart, c->d_step, 0, c->d_status, 0);
+    launch_cell_starts(c, 0);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos,
                        c->ucell, c->urank, c->cstart, a.type, a.pid, a.mol, b.pos, b.type, b.pid,
                        b.mol, c->slot_of, (int)c->n_cells, c->d_status);
