@@ -1923,19 +1923,21 @@ __global__ __launch_bounds__(CHUNK_THREADS) void k_cellscan(
   const int4 ch = chunks[b];
   const DRep& R = rep[ch.x];
   const int c0 = R.cbase + ch.y;
-  // particles in the replica's chunks before this one
-  int pre = 0;
-  for (int k = ch.w + t; k < b; k += CHUNK_THREADS) pre += csum[k];
-  const int base = R.pbase + block_sum_256(pre, s_red);
+  // this thread's cells first, then the particles in the replica's chunks
+  // before this one: the two sets of loads are in flight together
   constexpr int PER = CHUNK_CELLS / CHUNK_THREADS;
   int v[PER];
-  int tsum = 0;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int c = t * PER + i;
     v[i] = c < ch.z ? count[c0 + c] : 0;
-    tsum += v[i];
   }
+  int pre = 0;
+  for (int k = ch.w + t; k < b; k += CHUNK_THREADS) pre += csum[k];
+  int tsum = 0;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) tsum += v[i];
+  const int base = R.pbase + block_sum_256(pre, s_red);
   int inc = tsum;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
@@ -3116,6 +3118,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     // reaction worklist: a segment of 256 entries per slot block
     if ((rc = dalloc(&c->wl, sb.size() * 256)) != RDS_OK) return bail(rc);
     if ((rc = dalloc(&c->wl_n, sb.size())) != RDS_OK) return bail(rc);
+
     HIP_TRY(hipMemcpy(c->d_sblocks, sb.data(), sizeof(int4) * sb.size(), hipMemcpyHostToDevice));
   }
   HIP_TRY(hipMemset(c->count, 0, sizeof(int) * C));
