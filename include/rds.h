@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define RDS_ABI_VERSION 3
+#define RDS_ABI_VERSION 4
 
 #define RDS_MAX_TYPES 8
 #define RDS_MAX_TERMS 32
@@ -104,8 +104,8 @@ typedef struct {
 typedef struct {
   int32_t model;
   int32_t n_particles;
-  double L[2];
-  int32_t periodic[2];  /* 1 periodic, 0 reflecting (reference: periodic < 0) */
+  double L[3];          /* box; L[2] is read in three dimensions only */
+  int32_t periodic[3];  /* 1 periodic, 0 reflecting (reference: periodic < 0) */
   double force_prefac[RDS_MAX_TYPES];
   double noise_prefac[RDS_MAX_TYPES];
   double rate[RDS_MAX_REACTIONS];  /* rate of each reaction of the model */
@@ -144,7 +144,10 @@ typedef struct {
    * (0 = 64); a particle with more partners falls back to the stencil scan. */
   double skin;
   int32_t list_capacity;
-  int32_t reserved0;
+  /* 2 (0 means 2) or 3.  Three dimensions (pyrdsim.pyx:1159-1185,
+   * 1999-2067 loop over N_dim) run the cell search (skin = 0) in FP64 or
+   * mixed precision; the analysis and tiling entry points refuse them. */
+  int32_t n_dim;
   /* harmonic angles (pyrdsim.pyx:1322-1407, 2052-2065) in definition order,
    * global particle indices, host arrays; n_angles may be 0.  angle_j is the
    * vertex.  theta0 in rad and k0 per rad^2, as the reference holds them
@@ -231,6 +234,9 @@ int rds_compute_forces(rds_ctx* ctx, double* forces);
  * cells_xy [n_replicas][2] = (lx, ly) and cell_len [n_replicas][2] (host). */
 int rds_cell_ids(rds_ctx* ctx, int32_t* cells);
 int rds_grid(rds_ctx* ctx, int32_t* cells_xy, double* cell_len, int32_t* cell_base);
+/* The same for any n_dim: cells_xyz and cell_len [n_replicas][3] (host);
+ * lz = 1 in two dimensions. */
+int rds_grid3(rds_ctx* ctx, int32_t* cells_xyz, double* cell_len, int32_t* cell_base);
 
 /* Number of other particles (different molecule) within distance r, caller
  * order (the neighbour sets of create_neighbor_list, pyrdsim.pyx:1448-1587). */

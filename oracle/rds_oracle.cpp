@@ -127,10 +127,13 @@ inline double u53(uint32_t lo, uint32_t hi) {
 }
 
 // The engine's noise definition: counter = (particle index within the system,
-// step lo, step hi, stream 0), key = (seed lo, seed hi); Box-Muller in FP32 on
-// an odd 24-bit uniform in (0,1) and a 24-bit angle in [0,1).
-inline void philox_normals(uint64_t seed, int64_t step, int32_t i, double* n2) {
-  uint32_t ctr[4] = {(uint32_t)i, (uint32_t)step, (uint32_t)((uint64_t)step >> 32), 0u};
+// step lo, step hi, stream), key = (seed lo, seed hi); Box-Muller in FP32 on
+// an odd 24-bit uniform in (0,1) and a 24-bit angle in [0,1).  Stream 0 gives
+// the x and y normals; stream 2 gives the z normal (its first output) in three
+// dimensions; stream 1 is the reaction uniforms.
+inline void philox_normals(uint64_t seed, int64_t step, int32_t i, double* n2,
+                           uint32_t stream = 0u) {
+  uint32_t ctr[4] = {(uint32_t)i, (uint32_t)step, (uint32_t)((uint64_t)step >> 32), stream};
   uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
   uint32_t w[4];
   orc_philox4x32_10(ctr, key, w);
@@ -219,8 +222,8 @@ struct Oracle {
       return inj_normals[(o * n + j) * d + k];
     } else {
       double nn[2];
-      philox_normals(s.seed, step, j, nn);
-      return nn[k];
+      philox_normals(s.seed, step, j, nn, k < 2 ? 0u : 2u);
+      return nn[k & 1];
     }
   }
   double uniform(int64_t step, int32_t j, int which) {

@@ -15,7 +15,7 @@ MAX_TERMS = 32
 MAX_WALLS = 8
 MAX_REACTIONS = 8
 STATIC_LABELS_MAX = 8388608   # RDS_STATIC_LABELS_MAX
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 OK, E_ARG, E_HIP, E_BOUNDS, E_STATE, E_STREAM, E_ANGLE = 0, -1, -2, -3, -4, -5, -6
 FP64, MIXED, COMPENSATED = 0, 1, 2
@@ -59,8 +59,8 @@ class Model(C.Structure):
 
 
 class Replica(C.Structure):
-    _fields_ = [('model', C.c_int32), ('n_particles', C.c_int32), ('L', C.c_double * 2),
-                ('periodic', C.c_int32 * 2),
+    _fields_ = [('model', C.c_int32), ('n_particles', C.c_int32), ('L', C.c_double * 3),
+                ('periodic', C.c_int32 * 3),
                 ('force_prefac', C.c_double * MAX_TYPES),
                 ('noise_prefac', C.c_double * MAX_TYPES),
                 ('rate', C.c_double * MAX_REACTIONS), ('dt', C.c_double),
@@ -77,7 +77,7 @@ class Config(C.Structure):
                 ('n_bonds', C.c_int64), ('bond_i', C.POINTER(C.c_int32)),
                 ('bond_j', C.POINTER(C.c_int32)), ('bond_r0', C.POINTER(C.c_double)),
                 ('bond_u0', C.POINTER(C.c_double)), ('skin', C.c_double),
-                ('list_capacity', C.c_int32), ('reserved0', C.c_int32),
+                ('list_capacity', C.c_int32), ('n_dim', C.c_int32),
                 ('n_angles', C.c_int64), ('angle_i', C.POINTER(C.c_int32)),
                 ('angle_j', C.POINTER(C.c_int32)), ('angle_k', C.POINTER(C.c_int32)),
                 ('angle_theta0', C.POINTER(C.c_double)),
@@ -110,6 +110,7 @@ SIGNATURES = {
     'rds_compute_forces': (C.c_int, [_VP, _VP]),
     'rds_cell_ids': (C.c_int, [_VP, _VP]),
     'rds_grid': (C.c_int, [_VP, _VP, _VP, _VP]),
+    'rds_grid3': (C.c_int, [_VP, _VP, _VP, _VP]),
     'rds_neighbor_counts': (C.c_int, [_VP, C.c_double, _VP]),
     'rds_pair_histogram': (C.c_int, [_VP, C.c_double, C.c_double, C.c_int32, C.c_uint32, _VP]),
     'rds_type_counts': (C.c_int, [_VP, _VP]),

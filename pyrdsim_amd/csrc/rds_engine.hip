@@ -109,12 +109,14 @@ struct DModel {
 };
 
 struct DRep {
-  double L[2], cl[2];
-  double icl[2];          // 1 / cl, rounded
+  double L[3], cl[3];     // box and cell lengths per axis (z: three dimensions only)
+  double icl[3];          // 1 / cl, rounded
   int lx, ly, cbase, pbase;
   int n, model, px, py;   // periodic axes (boundary conditions)
   unsigned seed_lo, seed_hi;
   int sx, sy;             // periodic images in the cell stencil
+  int lz, pz, sz;         // the z axis (lz = 1 in two dimensions)
+  int ncell;              // lx * ly * lz
   double fpre[MAXT], npre[MAXT], rate[MAXRX];
   double dt;
   double rl2;             // list mode: (largest pair cutoff + skin)^2
@@ -195,6 +197,10 @@ struct StepArgs {
   // slot block (sblocks): block b's in wl[256 b ..), their number in wl_n[b]
   int* wl;
   int* wl_n;
+  // three dimensions: positions {x, y, z, 0}, next positions, bond forces
+  const double4* pos4;
+  double4* upos4;
+  const double4* fbond4;
 };
 
 // ---------------------------------------------------------------------------
@@ -218,10 +224,12 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, unsigned k0, unsigned k1
 
 // Two standard normals of particle `pid` (index within its replica) at
 // reference step `step`.
+// (Stream 0; the third normal of a three-dimensional particle comes from
+// stream 2, the same way.)
 __device__ __forceinline__ void philox_normals(const DRep& R, int pid, long long step,
-                                               float& n0, float& n1) {
+                                               float& n0, float& n1, unsigned stream = 0u) {
   uint4 w = philox4x32_10(make_uint4((unsigned)pid, (unsigned)step,
-                                     (unsigned)((unsigned long long)step >> 32), 0u),
+                                     (unsigned)((unsigned long long)step >> 32), stream),
                           R.seed_lo, R.seed_hi);
   float ua = (float)((w.x >> 9) * 2u + 1u) * 0x1.0p-24f;  // odd / 2^24, in (0,1)
   float ub = (float)(w.y >> 8) * 0x1.0p-24f;              // [0,1)
@@ -1959,7 +1967,7 @@ __global__ __launch_bounds__(CHUNK_THREADS) void k_cellscan(
     excl += v[i];
   }
   // the replica's last chunk checks the total
-  if (t == CHUNK_THREADS - 1 && ch.y + ch.z == R.lx * R.ly && excl != R.pbase + R.n)
+  if (t == CHUNK_THREADS - 1 && ch.y + ch.z == R.ncell && excl != R.pbase + R.n)
     atomicOr(&status->integrity, 2);
   if (advance && b == 0 && t == 0) *step = *step + 1;
 }
@@ -2180,6 +2188,386 @@ __global__ void k_cells_out(int n_total, const DRep* __restrict__ rep,
   out[p] = R.cbase + cell_coord(x.y, R.cl[1], R.ly) * R.lx + cell_coord(x.x, R.cl[0], R.lx);
 }
 
+// ---------------------------------------------------------------------------
+// three dimensions (rds_config.n_dim = 3), as the reference's loops over
+// N_dim (pyrdsim.pyx:1540, 1625-1647, 2244, 2274): positions double4
+// {x, y, z, 0} sorted by cell, cells (lx, ly, lz) at least one pair range
+// wide, and one thread per particle walking its 27-cell stencil straight from
+// memory, in the per-replica slot blocks (one replica per workgroup).  Cell
+// search only; FP64 or FP32 pair arithmetic.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int cell3(const DRep& R, double x, double y, double z) {
+  return R.cbase +
+         (cell_coord(z, R.cl[2], R.lz) * R.ly + cell_coord(y, R.cl[1], R.ly)) * R.lx +
+         cell_coord(x, R.cl[0], R.lx);
+}
+
+// f(j) for every slot j of the 27 cells around cell (cx, cy, cz); a
+// reflecting axis keeps to the box, a periodic one wraps (>= 3 cells)
+template <typename F>
+__device__ __forceinline__ void for_stencil3(const DRep& R, const int* __restrict__ cstart,
+                                             int cx, int cy, int cz, F f) {
+  for (int oz = -1; oz <= 1; ++oz) {
+    int z = cz + oz;
+    if (z < 0 || z >= R.lz) {
+      if (!R.sz) continue;
+      z = z < 0 ? z + R.lz : z - R.lz;
+    }
+    for (int oy = -1; oy <= 1; ++oy) {
+      int y = cy + oy;
+      if (y < 0 || y >= R.ly) {
+        if (!R.sy) continue;
+        y = y < 0 ? y + R.ly : y - R.ly;
+      }
+      for (int ox = -1; ox <= 1; ++ox) {
+        int x = cx + ox;
+        if (x < 0 || x >= R.lx) {
+          if (!R.sx) continue;
+          x = x < 0 ? x + R.lx : x - R.lx;
+        }
+        const int c = R.cbase + (z * R.ly + y) * R.lx + x;
+        const int j1 = cstart[c + 1];
+        for (int j = cstart[c]; j < j1; ++j) f(j);
+      }
+    }
+  }
+}
+
+// minimum-image displacement x_j - x_i on the periodic axes
+__device__ __forceinline__ double3 disp3(const DRep& R, double4 a, double4 b) {
+  double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+  if (R.px) dx = min_image(dx, R.L[0]);
+  if (R.py) dy = min_image(dy, R.L[1]);
+  if (R.pz) dz = min_image(dz, R.L[2]);
+  return make_double3(dx, dy, dz);
+}
+
+// Pair forces (every term of the type pair, file order), walls and bonds of
+// the particle in slot s; then, in MODE_STEP, the Euler-Maruyama update
+// (pyrdsim.pyx:2244-2250), the boundary conditions (pyrdsim.pyx:2269-2292)
+// and the binning of the new position.
+template <typename Real, bool MOL>
+__global__ __launch_bounds__(256) void k3_force(const StepArgs A, int mode) {
+  const int4 B = A.sblocks[blockIdx.x];
+  const DRep& Rp = A.rep[B.x];
+  const DModel& M = A.models[Rp.model];
+  const int s = B.y + (int)threadIdx.x;
+  if (s >= B.z) return;
+  const double4 xi = A.pos4[s];
+  const int ti = A.type[s], p = A.pid[s];
+  const int mi = MOL ? A.mol[s] : 0;
+  const int tbase = ti * M.nt;
+  Real fx = 0, fy = 0, fz = 0;
+  for_stencil3(Rp, A.cstart, cell_coord(xi.x, Rp.cl[0], Rp.lx), cell_coord(xi.y, Rp.cl[1], Rp.ly),
+               cell_coord(xi.z, Rp.cl[2], Rp.lz), [&](int j) {
+    if (j == s || (MOL && A.mol[j] == mi)) return;
+    const int pq = tbase + A.type[j];
+    const int q0 = M.pfirst[pq], qn = M.pcount[pq];
+    if (qn == 0) return;
+    const double3 d = disp3(Rp, xi, A.pos4[j]);
+    const Real rx = (Real)d.x, ry = (Real)d.y, rz = (Real)d.z;
+    const Real r2 = rx * rx + ry * ry + rz * rz;
+    if (!(r2 > Real(0))) return;  // coincident particles: no direction
+    for (int q = q0; q < q0 + qn; ++q) {
+      const DTerm& tm = M.terms[q];
+      if (!(r2 <= TP<Real>::rc2(tm))) continue;
+      const Real c = pair_coef<Real>(tm, r2);
+      fx += c * rx;
+      fy += c * ry;
+      fz += c * rz;
+    }
+  });
+  double F[3] = {(double)fx, (double)fy, (double)fz};
+  const double X[3] = {xi.x, xi.y, xi.z};
+  for (int q = 0; q < M.nwalls; ++q) {
+    const DWall& wl = M.walls[q];
+    if (wl.type == ti) F[wl.dir] += wall_force(wl, X[wl.dir]);
+  }
+  if (A.fbond4) {
+    const double4 fb = A.fbond4[s];
+    F[0] += fb.x;
+    F[1] += fb.y;
+    F[2] += fb.z;
+  }
+  if (mode == MODE_FORCE) {
+    double* out = reinterpret_cast<double*>(A.fout) + 3 * (size_t)p;
+    out[0] = F[0];
+    out[1] = F[1];
+    out[2] = F[2];
+    return;
+  }
+  const long long step_next = *A.step + 1;
+  double n[3];
+  if (A.rng == RDS_RNG_INJECTED) {
+    const long long o = step_next - A.inj_first;
+    if (o >= 0 && o < A.inj_count) {
+      const double* src = A.inj_n + (o * A.n_total + p) * 3;
+      n[0] = src[0];
+      n[1] = src[1];
+      n[2] = src[2];
+    } else {
+      n[0] = n[1] = n[2] = 0.0;
+      A.status->stream = 1;
+    }
+  } else {
+    float a0, a1, a2, a3;
+    philox_normals(Rp, p - Rp.pbase, step_next, a0, a1);
+    philox_normals(Rp, p - Rp.pbase, step_next, a2, a3, 2u);
+    n[0] = a0;
+    n[1] = a1;
+    n[2] = a2;
+  }
+  const double fp = Rp.fpre[ti];
+  const double np = A.include_rf * Rp.npre[ti];
+  const int per[3] = {Rp.px, Rp.py, Rp.pz};
+  double v[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double u = X[k];
+    {
+#pragma clang fp contract(off)
+      u = u + fp * F[k];
+      u = u + np * n[k];
+    }
+    const double L = Rp.L[k];
+    if (!per[k]) {
+      if (u > L) u -= 2.0 * (u - L);
+      if (u < 0.0) u -= 2.0 * u;
+    } else {
+      double m;
+      if (u > -L && u < L) m = u;
+      else if (u >= L && u < 2.0 * L) m = u - L;
+      else m = fmod(u, L);
+      if (m != 0.0 && ((m < 0.0) != (L < 0.0))) m += L;
+      u = m;
+    }
+    if (u < 0.0 || u > L || u != u)
+      record_bounds(A.status, ((unsigned long long)step_next << 32) |
+                                  (unsigned long long)(unsigned)(p * 3 + k), u);
+    v[k] = u;
+  }
+  const int c = cell3(Rp, v[0], v[1], v[2]);
+  A.upos4[s] = make_double4(v[0], v[1], v[2], 0.0);
+  A.ucell[s] = c;
+  A.urank[s] = atomicAdd(&A.count[c], 1);
+}
+
+// Harmonic bonds and angles in three dimensions (pyrdsim.pyx:1303-1407,
+// 2037-2065): the arithmetic of k_bonds with a third component.
+struct BondArgs3 {
+  int n_total;
+  const DRep* rep;
+  const int* rep_of;
+  const double4* pos;
+  const int* pid;
+  const int* slot_of;
+  const int* boff;
+  const int* bpart;
+  const double* br0;
+  const double* bu0;
+  const int* aoff;
+  const int2* aent;
+  const int4* apart;
+  const double2* aprm;
+  int angle_pi;
+  double angle_thr;
+  DStatus* status;
+  double4* fbond;
+};
+
+__global__ void k3_bonds(BondArgs3 A) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.n_total) return;
+  const int p = A.pid[s];
+  const DRep& R = A.rep[A.rep_of[p]];
+  const double4 xi = A.pos[s];
+  double f[3] = {0.0, 0.0, 0.0};
+  for (int b = A.boff[p]; b < A.boff[p + 1]; ++b) {
+    if (A.bu0[b] == 0.0) continue;
+    const double3 v = disp3(R, xi, A.pos[A.slot_of[A.bpart[b]]]);
+    const double d = sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+    const double g = A.bu0[b] * (d - A.br0[b]);
+    f[0] += g * (v.x / d);
+    f[1] += g * (v.y / d);
+    f[2] += g * (v.z / d);
+  }
+  if (A.aoff) {
+    for (int e = A.aoff[p]; e < A.aoff[p + 1]; ++e) {
+      const int2 en = A.aent[e];
+      const int4 gq = A.apart[en.x];
+      const double2 prm = A.aprm[en.x];
+      const double4 xv = A.pos[A.slot_of[gq.y]];
+      const double3 v1 = disp3(R, xv, A.pos[A.slot_of[gq.x]]);
+      const double3 v2 = disp3(R, xv, A.pos[A.slot_of[gq.z]]);
+      const double d1 = sqrt(v1.x * v1.x + v1.y * v1.y + v1.z * v1.z);
+      const double d2 = sqrt(v2.x * v2.x + v2.y * v2.y + v2.z * v2.z);
+      const double u1[3] = {v1.x / d1, v1.y / d1, v1.z / d1};
+      const double u2[3] = {v2.x / d2, v2.y / d2, v2.z / d2};
+      double h = u1[0] * u2[0] + u1[1] * u2[1] + u1[2] * u2[2];
+      double theta;
+      if (h < -1.0) {
+        if (h < -1.0000001 && atomicExch(&A.status->angle, 1) == 0) A.status->angle_val = h;
+        h = -1.0;
+        theta = M_PI;
+      } else {
+        theta = acos(h);
+      }
+      const double t = theta - prm.x;
+      double prefac;
+      if (A.angle_pi && !(theta < prm.x - A.angle_thr)) prefac = -prm.y / (1.0 - t * t / 6.0 + (t * t * t * t) / 120.0);
+      else prefac = prm.y * t / sqrt(1.0 - h * h);
+      const double c1 = prefac / d1, c3 = prefac / d2;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double f1 = c1 * (u2[k] - h * u1[k]), f3 = c3 * (u1[k] - h * u2[k]);
+        f[k] += en.y == 0 ? f1 : (en.y == 2 ? f3 : -(f1 + f3));
+      }
+    }
+  }
+  A.fbond[s] = make_double4(f[0], f[1], f[2], 0.0);
+}
+
+// Reorder of the new state into the other buffer, one thread per slot (the
+// types go to the pre-reaction column when reactions follow).
+__global__ void k3_scatter(int n_total, const double4* __restrict__ upos,
+                           const int* __restrict__ ucell, const int* __restrict__ urank,
+                           const int* __restrict__ cstart, const int* __restrict__ type,
+                           const int* __restrict__ pid, const int* __restrict__ mol,
+                           double4* __restrict__ npos, int* __restrict__ ntype,
+                           int* __restrict__ npid, int* __restrict__ nmol,
+                           int* __restrict__ slot_of, int n_cells, DStatus* status) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  const int c = ucell[s];
+  if (c < 0 || c >= n_cells) {
+    atomicOr(&status->integrity, 1);
+    return;
+  }
+  const int d = cstart[c] + urank[s];
+  if (d < 0 || d >= n_total) {
+    atomicOr(&status->integrity, 1);
+    return;
+  }
+  const int p = pid[s];
+  npos[d] = upos[s];
+  ntype[d] = type[s];
+  npid[d] = p;
+  if (nmol) nmol[d] = mol[s];
+  if (slot_of) slot_of[p] = d;
+}
+
+// The reaction step on the new positions with the old types
+// (pyrdsim.pyx:2070-2222), one thread per slot in the per-replica slot
+// blocks: neighbour counts over the 27-cell stencil for reactants of a
+// catalysed or inhibited reaction, then the decision.
+template <bool MOL>
+__global__ __launch_bounds__(256) void k3_react(const StepArgs A) {
+  const int4 B = A.sblocks[blockIdx.x];
+  const DRep& Rp = A.rep[B.x];
+  const DModel& M = A.models[Rp.model];
+  const int s = B.y + (int)threadIdx.x;
+  if (s >= B.z) return;
+  const int nrx = M.nrx;
+  const int ti = A.type[s];
+  if (nrx == 0) {
+    A.type_out[s] = ti;
+    return;
+  }
+  const int p = A.pid[s];
+  unsigned long long c_lo = 0ull, c_hi = 0ull;
+  if ((M.walk_types >> ti) & 1u) {
+    unsigned nbr_types = 0u;
+    for (int q = 0; q < nrx; ++q) {
+      const DRx& rx = M.rx[q];
+      if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
+      const int a = rx.kind == RDS_RX_CATALYZED ? rx.catalyst : rx.inh;
+      const int b = rx.kind == RDS_RX_CATALYZED ? -1 : rx.inh2;
+      if (a >= 0 && a < 32) nbr_types |= 1u << a;
+      if (b >= 0 && b < 32) nbr_types |= 1u << b;
+    }
+    const double4 xi = A.pos4[s];
+    const int mi = MOL ? A.mol[s] : 0;
+    for_stencil3(Rp, A.cstart, cell_coord(xi.x, Rp.cl[0], Rp.lx),
+                 cell_coord(xi.y, Rp.cl[1], Rp.ly), cell_coord(xi.z, Rp.cl[2], Rp.lz), [&](int j) {
+      if (j == s) return;
+      const int tj = A.type[j];
+      if (tj < 0 || tj >= 32 || !((nbr_types >> tj) & 1u)) return;
+      // the reference's neighbour list holds other molecules' particles and
+      // the bonded partners (pyrdsim.pyx:1513, 1523)
+      if (MOL && A.mol[j] == mi) {
+        bool bonded = false;
+        if (A.boff) {
+          const int pj = A.pid[j];
+          for (int b = A.boff[p]; b < A.boff[p + 1]; ++b) bonded |= A.bpart[b] == pj;
+        }
+        if (!bonded) return;
+      }
+      const double3 d = disp3(Rp, xi, A.pos4[j]);
+      const double d2 = d.x * d.x + d.y * d.y + d.z * d.z;
+#pragma unroll 1
+      for (int q = 0; q < nrx; ++q) {
+        const DRx& rx = M.rx[q];
+        if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
+        const bool match = (rx.kind == RDS_RX_CATALYZED) ? (tj == rx.catalyst)
+                                                         : (tj == rx.inh || tj == rx.inh2);
+        if (match && (d2 < rx.r0sq_lo || (d2 <= rx.r0sq_hi && sqrt(d2) < rx.r0)))
+          cnt_add(c_lo, c_hi, q);
+      }
+    });
+  }
+  A.type_out[s] = react_decide(A, Rp, M, ti, p, *A.step, c_lo, c_hi);
+}
+
+__global__ void k3_load(int n_total, const double* __restrict__ x, const int* __restrict__ t,
+                        const int* __restrict__ m, double4* __restrict__ pos,
+                        int* __restrict__ type, int* __restrict__ pid, int* __restrict__ mol) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  pos[s] = make_double4(x[3 * (size_t)s], x[3 * (size_t)s + 1], x[3 * (size_t)s + 2], 0.0);
+  type[s] = t[s];
+  pid[s] = s;
+  if (mol) mol[s] = m[s];
+}
+
+__global__ void k3_bin(int n_total, const DRep* __restrict__ rep, const int* __restrict__ rep_of,
+                       const double4* __restrict__ pos, int* __restrict__ ucell,
+                       int* __restrict__ urank, int* __restrict__ count) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  const DRep& R = rep[rep_of[s]];
+  const double4 x = pos[s];
+  const int c = cell3(R, x.x, x.y, x.z);
+  ucell[s] = c;
+  urank[s] = atomicAdd(&count[c], 1);
+}
+
+__global__ void k3_gather_out(int n_total, const double4* __restrict__ pos,
+                              const int* __restrict__ type, const int* __restrict__ pid,
+                              double* __restrict__ xo, int* __restrict__ to) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  const int p = pid[s];
+  if (xo) {
+    const double4 x = pos[s];
+    xo[3 * (size_t)p] = x.x;
+    xo[3 * (size_t)p + 1] = x.y;
+    xo[3 * (size_t)p + 2] = x.z;
+  }
+  if (to) to[p] = type[s];
+}
+
+__global__ void k3_cells_out(int n_total, const DRep* __restrict__ rep,
+                             const int* __restrict__ rep_of, const double4* __restrict__ pos,
+                             const int* __restrict__ pid, int* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_total) return;
+  const int p = pid[s];
+  const DRep& R = rep[rep_of[p]];
+  const double4 x = pos[s];
+  out[p] = cell3(R, x.x, x.y, x.z);
+}
+
 // Fills the LDS of every workgroup slot with a word pattern (test hook: NaN
 // patterns make later reads of never-written LDS visible).
 __global__ __launch_bounds__(256) void k_fill_lds(unsigned pattern, int words) {
@@ -2216,6 +2604,7 @@ static __device__ __align__(256) int scratch[RDS_STATIC_LABELS_MAX];
 
 struct Buf {
   double2* pos = nullptr;
+  double4* pos4 = nullptr;  // three dimensions
   int* type = nullptr;
   int* type0 = nullptr;  // pre-reaction types written by the scatter
   int* pid = nullptr;
@@ -2228,6 +2617,9 @@ struct rds_ctx {
   bool own_stream = true;
   int precision = RDS_MIXED, rng = RDS_RNG_PHILOX, prob_mode = RDS_PROB_LINEAR;
   int cpos = 0;  // compensated FP32 positions
+  int ndim = 2;
+  double4* upos4 = nullptr;   // three dimensions: next positions, bond forces
+  double4* fbond4 = nullptr;
   int has_mol = 0;
   int graph_steps = 0;
   double include_rf = 1.0;
@@ -2375,6 +2767,9 @@ StepArgs make_args(rds_ctx* c, const Buf& cur) {
   a.bpart = c->any_bonds ? c->bpart : nullptr;
   a.wl = c->wl;
   a.wl_n = c->wl_n;
+  a.pos4 = cur.pos4;
+  a.upos4 = c->upos4;
+  a.fbond4 = c->any_bonds ? c->fbond4 : nullptr;
   return a;
 }
 
@@ -2492,8 +2887,36 @@ struct Timer {
   bool on = false;
 };
 
-// bond and angle forces of the current state into fbond
+// bond and angle forces of the current state into fbond (fbond4 in three
+// dimensions)
+void launch_bonds3(rds_ctx* c, const Buf& cur) {
+  BondArgs3 b;
+  b.n_total = (int)c->n_total;
+  b.rep = c->d_rep;
+  b.rep_of = c->d_rep_of;
+  b.pos = cur.pos4;
+  b.pid = cur.pid;
+  b.slot_of = c->slot_of;
+  b.boff = c->boff;
+  b.bpart = c->bpart;
+  b.br0 = c->br0;
+  b.bu0 = c->bu0;
+  b.aoff = c->aoff;
+  b.aent = c->aent;
+  b.apart = c->apart;
+  b.aprm = c->aprm;
+  b.angle_pi = c->angle_pi;
+  b.angle_thr = c->angle_thr;
+  b.status = c->d_status;
+  b.fbond = c->fbond4;
+  hipLaunchKernelGGL(k3_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream, b);
+}
+
 void launch_bonds(rds_ctx* c, const Buf& cur) {
+  if (c->ndim == 3) {
+    launch_bonds3(c, cur);
+    return;
+  }
   BondArgs b;
   b.n_total = (int)c->n_total;
   b.rep = c->d_rep;
@@ -2515,6 +2938,17 @@ void launch_bonds(rds_ctx* c, const Buf& cur) {
   b.status = c->d_status;
   b.fbond = c->fbond;
   hipLaunchKernelGGL(k_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream, b);
+}
+
+void launch_force3(rds_ctx* c, const StepArgs& a, int mode) {
+  const dim3 g((unsigned)c->n_sblocks), b(256);
+  if (c->precision == RDS_FP64) {
+    if (c->has_mol) hipLaunchKernelGGL((k3_force<double, true>), g, b, 0, c->stream, a, mode);
+    else hipLaunchKernelGGL((k3_force<double, false>), g, b, 0, c->stream, a, mode);
+  } else {
+    if (c->has_mol) hipLaunchKernelGGL((k3_force<float, true>), g, b, 0, c->stream, a, mode);
+    else hipLaunchKernelGGL((k3_force<float, false>), g, b, 0, c->stream, a, mode);
+  }
 }
 
 // One integration step on the context's stream.  `ev` (optional) receives event
@@ -2540,6 +2974,34 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
     mark(KT_BONDS, false);
   }
   StepArgs a = make_args(c, cur);
+  if (c->ndim == 3) {
+    mark(KT_FORCE, true);
+    launch_force3(c, a, MODE_STEP);
+    mark(KT_FORCE, false);
+    mark(KT_SCAN, true);
+    launch_cell_starts(c, 1);
+    mark(KT_SCAN, false);
+    mark(KT_SCATTER, true);
+    hipLaunchKernelGGL(k3_scatter, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                       (int)c->n_total, c->upos4, c->ucell, c->urank, c->cstart, cur.type,
+                       cur.pid, cur.mol, nxt.pos4, c->any_rx ? nxt.type0 : nxt.type, nxt.pid,
+                       nxt.mol, c->any_bonds ? c->slot_of : nullptr, (int)c->n_cells,
+                       c->d_status);
+    mark(KT_SCATTER, false);
+    if (c->any_rx) {
+      StepArgs r = make_args(c, nxt);
+      r.type = nxt.type0;
+      r.type_out = nxt.type;
+      mark(KT_REACT, true);
+      if (c->has_mol)
+        hipLaunchKernelGGL((k3_react<true>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
+      else
+        hipLaunchKernelGGL((k3_react<false>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
+      mark(KT_REACT, false);
+    }
+    c->parity ^= 1;
+    return;
+  }
   if (c->list) {
     // positions move in place (same slots) into the other buffer; types and
     // the sort are touched only by a rebuild; reactions write the other type
@@ -2647,7 +3109,7 @@ int dalloc(T** p, size_t n) {
     if (rc_ != RDS_OK) return rc_; \
   } while (0)
 
-int fill_model(const rds_model& m, DModel& d, double* rcut) {
+int fill_model(const rds_model& m, DModel& d, double* rcut, int ndim) {
   memset(&d, 0, sizeof d);
   if (m.n_types < 1 || m.n_types > MAXT) return fail(RDS_E_ARG, "n_types %d outside [1, %d]", m.n_types, MAXT);
   if (m.n_terms < 0 || m.n_terms > MAXTERMS) return fail(RDS_E_ARG, "n_terms %d outside [0, %d]", m.n_terms, MAXTERMS);
@@ -2708,7 +3170,8 @@ int fill_model(const rds_model& m, DModel& d, double* rcut) {
   for (int q = 0; q < k; ++q) d.rc2max = std::max(d.rc2max, d.terms[q].rc2);
   for (int q = 0; q < m.n_walls; ++q) {
     const rds_wall_term& w = m.walls[q];
-    if (w.direction < 0 || w.direction > 1) return fail(RDS_E_ARG, "wall direction must be 0 or 1");
+    if (w.direction < 0 || w.direction >= ndim)
+      return fail(RDS_E_ARG, "wall direction must lie in [0, %d)", ndim);
     d.walls[q].type = w.particle_type;
     d.walls[q].dir = w.direction;
     d.walls[q].u0 = w.u0;
@@ -2809,6 +3272,7 @@ int rds_abi_version(void) { return RDS_ABI_VERSION; }
 
 int rds_set_tiling(rds_ctx* c, int32_t tile_rows, int32_t tile_particles) {
   if (!c) return fail(RDS_E_ARG, "null context");
+  if (c->ndim != 2) return fail(RDS_E_ARG, "three-dimensional contexts have no force tiles");
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (!build_tiles(c, tile_rows, tile_particles)) return RDS_E_ARG;
   RDS_TRY(upload_tiles(c));
@@ -2842,8 +3306,11 @@ int rds_ctx_destroy(rds_ctx* c) {
                              c->d_status, c->boff, c->bpart, c->br0, c->bu0, c->fbond,
                              c->slot_of, c->clabel, c->aoff, c->aent, c->apart, c->aprm, c->nl4, c->ncnt, c->xref, c->scell, c->spos,
                              c->stype, c->spid, c->smol};
+  ptrs.push_back(c->upos4);
+  ptrs.push_back(c->fbond4);
   for (int k = 0; k < 2; ++k) {
-    void* b[] = {c->buf[k].pos, c->buf[k].type, c->buf[k].type0, c->buf[k].pid, c->buf[k].mol};
+    void* b[] = {c->buf[k].pos, c->buf[k].pos4, c->buf[k].type, c->buf[k].type0, c->buf[k].pid,
+                 c->buf[k].mol};
     for (void* p : b) ptrs.push_back(p);
   }
   // list mode shares some columns between the two buffers: free each once
@@ -2910,6 +3377,11 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   };
   c->precision = cfg->precision;
   c->cpos = cfg->precision == RDS_COMPENSATED ? 1 : 0;
+  c->ndim = cfg->n_dim != 0 ? cfg->n_dim : 2;
+  if (c->ndim != 2 && c->ndim != 3) return bail(fail(RDS_E_ARG, "n_dim must be 2 or 3"));
+  if (c->ndim == 3 && (cfg->skin > 0.0 || cfg->precision == RDS_COMPENSATED))
+    return bail(fail(RDS_E_ARG,
+                     "three dimensions run the cell search (skin = 0) in FP64 or mixed precision"));
   c->rng = cfg->rng;
   c->prob_mode = cfg->reaction_prob;
   c->has_mol = cfg->has_molecules ? 1 : 0;
@@ -2935,7 +3407,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   std::vector<double> rcut(cfg->n_models);
   c->h_models.resize(cfg->n_models);
   for (int m = 0; m < cfg->n_models; ++m) {
-    rc = fill_model(cfg->models[m], c->h_models[m], &rcut[m]);
+    rc = fill_model(cfg->models[m], c->h_models[m], &rcut[m], c->ndim);
     if (rc) return bail(rc);
     if (cfg->models[m].n_reactions > 0) c->any_rx = true;
   }
@@ -2964,11 +3436,17 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     memset(&d, 0, sizeof d);
     if (s.model < 0 || s.model >= cfg->n_models) return bail(fail(RDS_E_ARG, "replica %d: bad model", r));
     if (s.n_particles < 1) return bail(fail(RDS_E_ARG, "replica %d: no particles", r));
+    const int nd = c->ndim;
     double rcm = rcut[s.model];
-    if (!(rcm > 0.0)) rcm = std::min(s.L[0], s.L[1]);  // no pair range: one cell suffices
-    else rcm += c->skin;                                // list mode: cells >= range + skin
-    int lcell[2];
-    for (int k = 0; k < 2; ++k) {
+    if (!(rcm > 0.0)) {  // no pair range: one cell suffices
+      rcm = std::min(s.L[0], s.L[1]);
+      if (nd == 3) rcm = std::min(rcm, s.L[2]);
+    } else {
+      rcm += c->skin;  // list mode: cells >= range + skin
+    }
+    int lcell[3] = {1, 1, 1};
+    d.L[2] = d.cl[2] = d.icl[2] = 1.0;
+    for (int k = 0; k < nd; ++k) {
       if (!(s.L[k] > 0.0)) return bail(fail(RDS_E_ARG, "replica %d: box length must be > 0", r));
       // largest l with L/l >= rcut (the reference's rule, pyrdsim.pyx:1439-1441,
       // with rs = the largest interaction range and ratio = 1)
@@ -2994,6 +3472,10 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     d.py = s.periodic[1] ? 1 : 0;
     d.sx = (d.px && d.lx >= 3) ? 1 : 0;
     d.sy = (d.py && d.ly >= 3) ? 1 : 0;
+    d.lz = lcell[2];
+    d.pz = (nd == 3 && s.periodic[2]) ? 1 : 0;
+    d.sz = (d.pz && d.lz >= 3) ? 1 : 0;
+    d.ncell = d.lx * d.ly * d.lz;
     d.cbase = (int)cbase;
     d.pbase = (int)pbase;
     d.n = s.n_particles;
@@ -3021,13 +3503,14 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
       d.rl2 = rl * rl;
     }
     pbase += s.n_particles;
-    cbase += (int64_t)d.lx * d.ly;
+    cbase += (int64_t)d.ncell;
     if (pbase > 0x7fffffff || cbase > 0x7ffffff0)
       return bail(fail(RDS_E_ARG, "too many particles or cells for 32-bit indices"));
   }
   c->n_total = pbase;
   c->n_cells = cbase;
-  if (!build_tiles(c, c->rows, (int)target)) return bail(RDS_E_ARG);
+  // force tiles (two dimensions; the three-dimensional kernels walk per slot)
+  if (c->ndim == 2 && !build_tiles(c, c->rows, (int)target)) return bail(RDS_E_ARG);
 
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   if ((rc = scratch_ptr(c->device, &c->static_labels)) != RDS_OK) return bail(rc);
@@ -3044,7 +3527,16 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   A_(c->d_rep, c->n_rep);
   A_(c->d_models, c->n_models);
   A_(c->d_rep_of, N);
-  if (!c->list) {
+  if (c->ndim == 3) {
+    for (int k = 0; k < 2; ++k) {
+      A_(c->buf[k].pos4, N);
+      A_(c->buf[k].type, N);
+      A_(c->buf[k].type0, N);
+      A_(c->buf[k].pid, N);
+      if (c->has_mol) A_(c->buf[k].mol, N);
+    }
+    A_(c->upos4, N);
+  } else if (!c->list) {
     for (int k = 0; k < 2; ++k) {
       A_(c->buf[k].pos, N);
       A_(c->buf[k].type, N);
@@ -3087,7 +3579,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     std::vector<int4> ch;
     for (int r = 0; r < c->n_rep; ++r) {
       const DRep& d = c->h_rep[r];
-      const int cr = d.lx * d.ly, first = (int)ch.size();
+      const int cr = d.ncell, first = (int)ch.size();
       for (int o = 0; o < cr; o += CHUNK_CELLS)
         ch.push_back(make_int4(r, o, std::min(CHUNK_CELLS, cr - o), first));
     }
@@ -3165,7 +3657,8 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     A_(c->bpart, part.size());
     A_(c->br0, r0v.size());
     A_(c->bu0, u0v.size());
-    A_(c->fbond, N);
+    if (c->ndim == 3) A_(c->fbond4, N);
+    else A_(c->fbond, N);
     HIP_TRY(hipMemcpy(c->boff, deg.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->bpart, part.data(), sizeof(int) * part.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(c->br0, r0v.data(), sizeof(double) * r0v.size(), hipMemcpyHostToDevice));
@@ -3221,14 +3714,26 @@ int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32
   c->parity = 0;
   Buf& a = c->buf[0];
   Buf& b = c->buf[1];
-  hipLaunchKernelGGL(k_load, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, (const double2*)x,
-                     types, c->has_mol ? mol : nullptr, a.pos, a.type, a.pid,
-                     c->has_mol ? a.mol : nullptr, c->cpos);
+  if (c->ndim == 2)
+    hipLaunchKernelGGL(k_load, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N,
+                       (const double2*)x, types, c->has_mol ? mol : nullptr, a.pos, a.type,
+                       a.pid, c->has_mol ? a.mol : nullptr, c->cpos);
   DStatus st;
   memset(&st, 0, sizeof st);
   st.err_key = ~0ull;
   HIP_TRY(hipMemcpyAsync(c->d_status, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
-  if (c->list) {
+  if (c->ndim == 3) {
+    hipLaunchKernelGGL(k3_load, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, x, types,
+                       c->has_mol ? mol : nullptr, a.pos4, a.type, a.pid,
+                       c->has_mol ? a.mol : nullptr);
+    hipLaunchKernelGGL(k3_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep,
+                       c->d_rep_of, a.pos4, c->ucell, c->urank, c->count);
+    launch_cell_starts(c, 0);
+    hipLaunchKernelGGL(k3_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos4,
+                       c->ucell, c->urank, c->cstart, a.type, a.pid, a.mol, b.pos4, b.type,
+                       b.pid, b.mol, c->slot_of, (int)c->n_cells, c->d_status);
+    c->parity = 1;
+  } else if (c->list) {
     // sorted in place, lists built
     force_rebuild(c);
   } else {
@@ -3255,8 +3760,12 @@ int rds_get_state(rds_ctx* c, double* x, int32_t* types) {
   if (!c) return fail(RDS_E_ARG, "null context");
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   Buf& cur = c->buf[c->parity];
-  hipLaunchKernelGGL(k_gather_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
-                     (int)c->n_total, cur.pos, cur.type, cur.pid, (double2*)x, types, c->cpos);
+  if (c->ndim == 3)
+    hipLaunchKernelGGL(k3_gather_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                       (int)c->n_total, cur.pos4, cur.type, cur.pid, x, types);
+  else
+    hipLaunchKernelGGL(k_gather_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                       (int)c->n_total, cur.pos, cur.type, cur.pid, (double2*)x, types, c->cpos);
   HIP_TRY(hipGetLastError());
   return RDS_OK;
 }
@@ -3321,8 +3830,8 @@ int rds_sync(rds_ctx* c, rds_status* out) {
   if (st.bounds) {
     s.error_step = (int64_t)(st.err_key >> 32);
     unsigned lo = (unsigned)(st.err_key & 0xffffffffu);
-    s.error_particle = (int32_t)(lo / 2);
-    s.error_axis = (int32_t)(lo % 2);
+    s.error_particle = (int32_t)(lo / (unsigned)c->ndim);
+    s.error_axis = (int32_t)(lo % (unsigned)c->ndim);
     s.error_value = st.err_val;
   }
   if (out) *out = s;
@@ -3352,7 +3861,8 @@ int rds_compute_forces(rds_ctx* c, double* forces) {
     launch_bonds(c, cur);
   StepArgs a = make_args(c, cur);
   a.fout = (double2*)forces;
-  launch_force(c, a, MODE_FORCE);
+  if (c->ndim == 3) launch_force3(c, a, MODE_FORCE);
+  else launch_force(c, a, MODE_FORCE);
   HIP_TRY(hipGetLastError());
   return RDS_OK;
 }
@@ -3361,8 +3871,12 @@ int rds_cell_ids(rds_ctx* c, int32_t* cells) {
   if (!c || !cells) return fail(RDS_E_ARG, "null argument");
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   Buf& cur = c->buf[c->parity];
-  hipLaunchKernelGGL(k_cells_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
-                     (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, cells, c->cpos);
+  if (c->ndim == 3)
+    hipLaunchKernelGGL(k3_cells_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                       (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos4, cur.pid, cells);
+  else
+    hipLaunchKernelGGL(k_cells_out, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
+                       (int)c->n_total, c->d_rep, c->d_rep_of, cur.pos, cur.pid, cells, c->cpos);
   HIP_TRY(hipGetLastError());
   return RDS_OK;
 }
@@ -3383,8 +3897,31 @@ int rds_grid(rds_ctx* c, int32_t* cells_xy, double* cell_len, int32_t* cell_base
   return RDS_OK;
 }
 
+int rds_grid3(rds_ctx* c, int32_t* cells_xyz, double* cell_len, int32_t* cell_base) {
+  if (!c) return fail(RDS_E_ARG, "null context");
+  for (int r = 0; r < c->n_rep; ++r) {
+    const DRep& d = c->h_rep[r];
+    if (cells_xyz) {
+      cells_xyz[3 * r] = d.lx;
+      cells_xyz[3 * r + 1] = d.ly;
+      cells_xyz[3 * r + 2] = d.lz;
+    }
+    if (cell_len)
+      for (int k = 0; k < 3; ++k) cell_len[3 * r + k] = d.cl[k];
+    if (cell_base) cell_base[r] = d.cbase;
+  }
+  return RDS_OK;
+}
+
+// Entry points built for two dimensions only.
+static int two_d_only(rds_ctx* c, const char* what) {
+  if (c->ndim != 2) return fail(RDS_E_ARG, "%s works on two-dimensional systems", what);
+  return RDS_OK;
+}
+
 int rds_neighbor_counts(rds_ctx* c, double r, int32_t* counts) {
   if (!c || !counts) return fail(RDS_E_ARG, "null argument");
+  if (int e = two_d_only(c, "rds_neighbor_counts")) return e;
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   for (int q = 0; q < c->n_rep; ++q) {
     const DRep& d = c->h_rep[q];
@@ -3419,6 +3956,7 @@ static int check_radius(rds_ctx* c, double r, const char* what) {
 int rds_pair_histogram(rds_ctx* c, double r_max, double dr, int32_t n_bins, uint32_t type_mask,
                        uint64_t* hist) {
   if (!c || !hist) return fail(RDS_E_ARG, "null argument");
+  if (int e = two_d_only(c, "rds_pair_histogram")) return e;
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   if (int e = check_radius(c, r_max, "r_max")) return e;
   if (!(dr > 0.0) || n_bins < 1 || n_bins > 4096)
@@ -3477,6 +4015,7 @@ static int cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t
 
 int rds_cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* labels) {
   if (!c || !labels) return fail(RDS_E_ARG, "null argument");
+  if (int e = two_d_only(c, "rds_cluster_labels")) return e;
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   if (int e = check_radius(c, r_link, "r_link")) return e;
   if (!c->clabel && dalloc(&c->clabel, c->n_total) != RDS_OK) return RDS_E_HIP;
@@ -3485,6 +4024,7 @@ int rds_cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* l
 
 int rds_cluster_labels_static(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* labels) {
   if (!c || !labels) return fail(RDS_E_ARG, "null argument");
+  if (int e = two_d_only(c, "rds_cluster_labels_static")) return e;
   if (c->n_total > RDS_STATIC_LABELS_MAX)
     return fail(RDS_E_ARG,
                 "rds_cluster_labels_static handles at most %d particles (context has %lld); "

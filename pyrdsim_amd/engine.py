@@ -105,10 +105,14 @@ def prefactors(dt, tau_by_tauD):
 
 
 def make_replica(model_index, n_particles, L_box, periodic, dt, tau_by_tauD, rates, seed):
+    """One replica; L_box and periodic have one entry per axis (2 or 3)."""
     r = L.Replica()
     r.model = model_index
     r.n_particles = int(n_particles)
-    for k in range(2):
+    nd = len(L_box)
+    if nd not in (2, 3) or len(periodic) < nd:
+        raise RuntimeError('L_box needs 2 or 3 axes and periodic one entry per axis')
+    for k in range(nd):
         r.L[k] = float(L_box[k])
         pk = float(periodic[k])
         if pk == 0.0 or pk != pk:
@@ -143,19 +147,23 @@ class Engine:
     bonds = (i, j, r0, u0) and angles = (i1, i2, i3, theta0 [rad], k0 [per
     rad^2]) are arrays over global particle indices, in definition order
     (pyrdsim.pyx:1287-1407); i2 is an angle's vertex.
+
+    n_dim = 3 integrates three-dimensional replicas (cell search, 'fp64' or
+    'mixed'); positions, forces and injected normals then carry three
+    components.
     """
 
     def __init__(self, models, replicas, precision='mixed', rng='philox',
                  reaction_prob='linear', bonds=None, has_molecules=False,
                  tile_particles=0, graph_steps=0, tile_rows=1, device='cuda',
-                 skin=0.0, list_capacity=0, angles=None, angle_threshold=0.1):
+                 skin=0.0, list_capacity=0, angles=None, angle_threshold=0.1, n_dim=2):
         self._init = (list(models), list(replicas),
                       dict(precision=precision, rng=rng, reaction_prob=reaction_prob,
                            bonds=bonds, has_molecules=has_molecules,
                            tile_particles=tile_particles, graph_steps=graph_steps,
                            tile_rows=tile_rows, device=device, skin=skin,
                            list_capacity=list_capacity, angles=angles,
-                           angle_threshold=angle_threshold))
+                           angle_threshold=angle_threshold, n_dim=n_dim))
         self.lib = L.load()
         if not torch.cuda.is_available():
             raise RuntimeError('the engine needs a ROCm GPU; none is visible')
@@ -182,6 +190,10 @@ class Engine:
         cfg.tile_rows = int(tile_rows)
         cfg.skin = float(skin)
         cfg.list_capacity = int(list_capacity)
+        if int(n_dim) not in (2, 3):
+            raise RuntimeError('n_dim must be 2 or 3')
+        cfg.n_dim = int(n_dim)
+        self.n_dim = int(n_dim)
         self.skin = float(skin)
         self._bond_arrays = None
         if bonds is not None and len(bonds[0]) > 0:
@@ -249,8 +261,8 @@ class Engine:
         return t.to(self.device, non_blocking=False).contiguous()
 
     def set_state(self, x, types, molecules=None, step0=0):
-        """Positions [N][2] f64, types [N], molecule numbers [N] (caller order)."""
-        x = np.asarray(x, dtype=np.float64).reshape(self.n_total, 2)
+        """Positions [N][n_dim] f64, types [N], molecule numbers [N] (caller order)."""
+        x = np.asarray(x, dtype=np.float64).reshape(self.n_total, self.n_dim)
         types = np.asarray(types).reshape(-1)
         lo = 0
         for r in self.replicas:
@@ -272,13 +284,13 @@ class Engine:
         self.step = int(step0)
 
     def set_injected(self, normals, u1, u2=None, first_step=1):
-        """Injected streams: normals [S][N][2], u1/u2 [S][N] for steps
+        """Injected streams: normals [S][N][n_dim], u1/u2 [S][N] for steps
         first_step .. first_step+S-1 (the reference's step numbers)."""
         n = self._dev(np.asarray(normals, dtype=np.float64), torch.float64)
         a = self._dev(np.asarray(u1, dtype=np.float64), torch.float64)
         b = self._dev(np.asarray(u2, dtype=np.float64), torch.float64) if u2 is not None else None
         S = n.shape[0]
-        if n.numel() != S * self.n_total * 2 or a.numel() != S * self.n_total or \
+        if n.numel() != S * self.n_total * self.n_dim or a.numel() != S * self.n_total or \
                 (b is not None and b.numel() != S * self.n_total):
             raise ValueError('injected stream shapes do not match the particle count')
         torch.cuda.synchronize(self.device)
@@ -306,7 +318,7 @@ class Engine:
 
     # -- outputs ----------------------------------------------------------
     def state(self):
-        x = torch.empty((self.n_total, 2), dtype=torch.float64, device=self.device)
+        x = torch.empty((self.n_total, self.n_dim), dtype=torch.float64, device=self.device)
         t = torch.empty((self.n_total,), dtype=torch.int32, device=self.device)
         L.check(self.lib.rds_get_state(self.ctx, x.data_ptr(), t.data_ptr()))
         self.sync()
@@ -317,7 +329,7 @@ class Engine:
         L.check(self.lib.rds_get_state(self.ctx, x.data_ptr(), t.data_ptr()))
 
     def forces(self):
-        f = torch.empty((self.n_total, 2), dtype=torch.float64, device=self.device)
+        f = torch.empty((self.n_total, self.n_dim), dtype=torch.float64, device=self.device)
         L.check(self.lib.rds_compute_forces(self.ctx, f.data_ptr()))
         self.sync()
         return f.cpu().numpy()
@@ -329,12 +341,13 @@ class Engine:
         return c.cpu().numpy()
 
     def grid(self):
+        """Cells per axis [R][n_dim], cell lengths [R][n_dim], first global cell [R]."""
         n = len(self.replicas)
-        lxy = np.zeros((n, 2), dtype=np.int32)
-        cl = np.zeros((n, 2), dtype=np.float64)
+        lxyz = np.zeros((n, 3), dtype=np.int32)
+        cl = np.zeros((n, 3), dtype=np.float64)
         base = np.zeros((n,), dtype=np.int32)
-        L.check(self.lib.rds_grid(self.ctx, lxy.ctypes.data, cl.ctypes.data, base.ctypes.data))
-        return lxy, cl, base
+        L.check(self.lib.rds_grid3(self.ctx, lxyz.ctypes.data, cl.ctypes.data, base.ctypes.data))
+        return lxyz[:, :self.n_dim].copy(), cl[:, :self.n_dim].copy(), base
 
     def neighbor_counts(self, r):
         c = torch.empty((self.n_total,), dtype=torch.int32, device=self.device)

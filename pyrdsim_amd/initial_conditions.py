@@ -13,13 +13,15 @@ Here, everything is driven by one seeded ``numpy.random.Generator``:
   reference does) or found by minimising the reference's bonded energy
   (pyrdsim.pyx:2496-2527) from several seeded random starts with SciPy's
   BFGS (the ``cma`` package the reference uses is not available);
-* each molecule is a rigid copy of its geometry, rotated by a uniform random
-  angle, moved into the box and translated uniformly, as in
+* each molecule is a rigid copy of its geometry, rotated uniformly at random
+  (a uniform angle in 2-D, a Haar-random rotation in 3-D), moved into the box and translated uniformly, as in
   ``place_configuration_randomly`` (pyrdsim.pyx:2576-2611), and accepted only
   if every atom is at least ``min_distance`` from every placed atom (periodic
   axes by minimum image);
 * the relaxation runs once, on the GPU engine, after all molecules are placed.
 """
+
+import itertools
 
 import numpy as np
 
@@ -68,30 +70,48 @@ def reference_molecule(interactions, n_atoms, n_dim, rng, n_tries=10):
     return best
 
 
+def random_rotation(d, rng):
+    """A rotation drawn uniformly (Haar measure) from SO(d): in two dimensions
+    a uniform angle; in general the Q factor of a Gaussian matrix with the
+    signs of R's diagonal folded in (Mezzadri, "How to generate random
+    matrices from the classical compact groups", Notices AMS 54 (2007)),
+    its determinant made +1."""
+    if d == 2:
+        a = rng.random() * 2 * np.pi
+        c, s = np.cos(a), np.sin(a)
+        return np.array([[c, -s], [s, c]])
+    q, r = np.linalg.qr(rng.standard_normal((d, d)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
 def place_randomly(config, L, rng):
-    """A rigid copy of config rotated by a uniform angle, shifted so that its
+    """A rigid copy of config rotated uniformly at random, shifted so that its
     minimum is at 0 on each axis and translated by U(0, L - extent)
-    (pyrdsim.pyx:2576-2611).  Two-dimensional."""
-    a = rng.random() * 2 * np.pi
-    c, s = np.cos(a), np.sin(a)
-    out = config @ np.array([[c, s], [-s, c]])
+    (pyrdsim.pyx:2576-2611), in two or three dimensions."""
+    config = np.asarray(config, dtype=float)
+    d = config.shape[1]
+    out = config @ random_rotation(d, rng).T
     out -= np.minimum(out.min(axis=0), 0.0)
     ext = out.max(axis=0)
     if np.any(ext > L):
         raise RuntimeError('a molecule is larger than the box')
-    return out + rng.random(2) * (L - ext)
+    return out + rng.random(d) * (L - ext)
 
 
 class _Grid:
-    """Atoms placed so far, hashed into square bins of side >= min_distance."""
+    """Atoms placed so far, hashed into bins of side >= min_distance."""
 
     def __init__(self, L, periodic, h):
         self.L = np.asarray(L, dtype=float)
-        self.per = np.asarray(periodic) > 0
+        self.per = np.asarray(periodic)[:len(self.L)] > 0
         self.n = np.maximum((self.L // h).astype(int), 1)
         self.h = self.L / self.n
         self.bins = {}
         self.pts = []
+        self.offsets = list(itertools.product((-1, 0, 1), repeat=len(self.L)))
 
     def _bin(self, p):
         return tuple(np.minimum((p // self.h).astype(int), self.n - 1))
@@ -100,23 +120,22 @@ class _Grid:
         best = np.inf
         for p in atoms:
             b = self._bin(p)
-            for dx in (-1, 0, 1):
-                for dy in (-1, 0, 1):
-                    q = [b[0] + dx, b[1] + dy]
-                    skip = False
-                    for k in range(2):
-                        if q[k] < 0 or q[k] >= self.n[k]:
-                            if not self.per[k]:
-                                skip = True
-                            q[k] %= self.n[k]
-                    if skip:
-                        continue
-                    idx = self.bins.get(tuple(q))
-                    if not idx:
-                        continue
-                    d = np.asarray([self.pts[i] for i in idx]) - p
-                    d -= self.L * np.round(d / self.L) * self.per
-                    best = min(best, float(np.sqrt((d * d).sum(axis=1)).min()))
+            for off in self.offsets:
+                q = [b[k] + off[k] for k in range(len(b))]
+                skip = False
+                for k in range(len(q)):
+                    if q[k] < 0 or q[k] >= self.n[k]:
+                        if not self.per[k]:
+                            skip = True
+                        q[k] %= self.n[k]
+                if skip:
+                    continue
+                idx = self.bins.get(tuple(q))
+                if not idx:
+                    continue
+                d = np.asarray([self.pts[i] for i in idx]) - p
+                d -= self.L * np.round(d / self.L) * self.per
+                best = min(best, float(np.sqrt((d * d).sum(axis=1)).min()))
         return best
 
     def add(self, atoms):
@@ -129,11 +148,11 @@ def place_molecules(counts, particle_types, refs, L, periodic, min_distance, rng
                     n_tries=1000):
     """Output array of the reference's layout (pyrdsim.pyx:2711-2712): one row
     per atom, (molecule number, index in molecule, molecule id, particle type,
-    x, y), molecules in a seeded random order of their ids."""
+    x, y[, z]), molecules in a seeded random order of their ids."""
     order = np.concatenate([np.full(int(n), int(mid)) for mid, n in counts.items()])
     rng.shuffle(order)
     total = int(sum(len(refs[m]) for m in order))
-    out = np.zeros((total, 6))
+    out = np.zeros((total, 4 + len(L)))
     grid = _Grid(L, periodic, max(min_distance, 1e-9))
     row = 0
     for i, mid in enumerate(order):

@@ -302,8 +302,9 @@ class system:
         number, index in molecule, molecule id, type, positions) and writes
         initial_conditions_filename.  seed=None uses the system's seed."""
         from . import initial_conditions as icg
-        if self.N_dim != 2:
-            raise NotImplementedError('the MI355X engine integrates two-dimensional systems')
+        if self.N_dim not in (2, 3):
+            raise NotImplementedError('the MI355X engine integrates two- and three-dimensional '
+                                      'systems')
         rng = np.random.Generator(np.random.PCG64(self.seed if seed is None else seed))
         if min_distance_threshold is None:
             min_distance_threshold = self.rs / 4.
@@ -388,8 +389,9 @@ class system:
 
     # -- 4. simulation -----------------------------------------------------
     def _engine(self, random_force):
-        if self.N_dim != 2:
-            raise NotImplementedError('the MI355X engine integrates two-dimensional systems')
+        if self.N_dim not in (2, 3):
+            raise NotImplementedError('the MI355X engine integrates two- and three-dimensional '
+                                      'systems')
         model, rates = build_model(self.interactions, self.reactions, self.N_species)
         rep = make_replica(0, self.N_particles, self.L, self.periodic, self.dt,
                            self.tau_by_tauD, rates, self.seed)
@@ -400,7 +402,8 @@ class system:
                      bonds=bonds if len(bonds[0]) else None, has_molecules=has_mol,
                      graph_steps=int(self.graph_steps), skin=float(self.skin),
                      angles=angles if len(angles[0]) else None,
-                     angle_threshold=self.angle_pi_approximation_threshold)
+                     angle_threshold=self.angle_pi_approximation_threshold,
+                     n_dim=self.N_dim)
         eng.set_random_force(random_force)
         return eng
 

@@ -24,6 +24,10 @@ Five configurations (all 2-D, kT = 1):
 
 Initial positions are uniform random with overlap rejection at 0.9 sigma
 (random sequential addition), from a NumPy PCG64 seed.
+
+Three-dimensional variants (``lj3``, ``wall3``, ``patchy3``) take the same
+models in a cube, with positions on a randomly occupied simple-cubic lattice
+jittered by up to 10% of its spacing.
 """
 
 import math
@@ -267,3 +271,86 @@ def ensemble_systems(plan, seed=1005):
 
 
 BY_NAME = {'lj1': lj1, 'kalj': kalj, 'wall': wall, 'patchy': patchy}
+
+
+# ---------------------------------------------------------------------------
+# three-dimensional systems (the reference's N_dim = 3)
+# ---------------------------------------------------------------------------
+
+def lattice_positions3(n, L, rng, jitter=0.1):
+    """n points in the cube [0, L)^3: n random sites of the smallest simple
+    cubic lattice with >= n sites, each moved by U(-jitter, jitter) spacings
+    per axis; pair distances are >= (1 - 2 jitter) spacings."""
+    m = int(math.ceil(n ** (1.0 / 3.0) - 1e-9))
+    while m ** 3 < n:
+        m += 1
+    a = float(L) / m
+    site = rng.permutation(m ** 3)[:n]
+    idx = np.stack([site % m, (site // m) % m, site // (m * m)], axis=1).astype(float)
+    return (idx + 0.5 + rng.uniform(-jitter, jitter, (n, 3))) * a
+
+
+def lj3(n=4096, density=0.6, seed=2001, dt=5e-5):
+    """The two-species LJ mixture of ``kalj`` in a periodic cube."""
+    L = (n / density) ** (1.0 / 3.0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = lattice_positions3(n, L, rng)
+    types = np.zeros(n, dtype=np.intc)
+    types[rng.permutation(n)[: n // 2]] = 1
+    return dict(name='lj3', x=x, types=types, molecule_ids=_atoms(x, types),
+                L=np.array([L, L, L]), periodic=np.array([1.0, 1.0, 1.0]), dt=dt,
+                n_steps=10000, model=kalj_model(), bonds=None)
+
+
+def wall3(n=4096, density=0.4, seed=2002, k_a=0.1, k_d=1.0, dt=1e-4):
+    """The ``wall`` model in a cube periodic in x and z and reflecting in y
+    (the wall well at y = 0)."""
+    L = (n / density) ** (1.0 / 3.0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    x = lattice_positions3(n, L, rng)
+    types = np.zeros(n, dtype=np.intc)
+    return dict(name='wall3', x=x, types=types, molecule_ids=_atoms(x, types),
+                L=np.array([L, L, L]), periodic=np.array([1.0, -1.0, 1.0]), dt=dt,
+                n_steps=20000, model=wall_model(k_a, k_d), bonds=None)
+
+
+def patchy3(n_mol=1024, density=0.15, seed=2003, dt=1e-5, angle_k0=50.0):
+    """The ``patchy`` molecules in a periodic cube: three patches at radius 0.5
+    and 120 degrees in a random plane through the core, the six springs of
+    ``patchy``, and three patch-core-patch angles of 120 degrees (k0 per
+    rad^2) in definition order."""
+    L = (n_mol / density) ** (1.0 / 3.0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cores = lattice_positions3(n_mol, L, rng)
+    # an orthonormal pair (u, v) per molecule: the patch plane
+    g = rng.standard_normal((n_mol, 3))
+    u = g / np.linalg.norm(g, axis=1, keepdims=True)
+    h = rng.standard_normal((n_mol, 3))
+    h -= (h * u).sum(axis=1, keepdims=True) * u
+    v = h / np.linalg.norm(h, axis=1, keepdims=True)
+    x = np.zeros((4 * n_mol, 3))
+    types = np.zeros(4 * n_mol, dtype=np.intc)
+    mol = np.zeros((4 * n_mol, 3), dtype=np.intc)
+    for k in range(4):
+        sl = slice(k, 4 * n_mol, 4)
+        if k == 0:
+            x[sl] = cores
+        else:
+            a = 2 * math.pi * (k - 1) / 3.0
+            x[sl] = cores + 0.5 * (math.cos(a) * u + math.sin(a) * v)
+            types[sl] = 1
+        mol[sl, 0] = np.arange(n_mol)
+        mol[sl, 1] = k
+    x %= L
+    per = [(0, 1, 0.5), (0, 2, 0.5), (0, 3, 0.5),
+           (1, 2, 0.5 * math.sqrt(3)), (2, 3, 0.5 * math.sqrt(3)), (3, 1, 0.5 * math.sqrt(3))]
+    base = 4 * np.arange(n_mol)
+    bonds = tuple(np.concatenate(c) for c in zip(*[
+        (base + a, base + b, np.full(n_mol, r0), np.full(n_mol, 1000.0)) for a, b, r0 in per]))
+    tri = [(1, 0, 2), (2, 0, 3), (3, 0, 1)]
+    angles = tuple(np.concatenate(c) for c in zip(*[
+        (base + i, base + j, base + k, np.full(n_mol, 2 * math.pi / 3),
+         np.full(n_mol, float(angle_k0))) for i, j, k in tri]))
+    return dict(name='patchy3', x=x, types=types, molecule_ids=mol,
+                L=np.array([L, L, L]), periodic=np.array([1.0, 1.0, 1.0]), dt=dt,
+                n_steps=10000, model=patchy_model(), bonds=bonds, angles=angles)

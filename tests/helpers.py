@@ -19,6 +19,15 @@ def small(name, **kw):
     if name == 'patchy':
         kw.setdefault('n_mol', 256)
         return workloads.patchy(**kw)
+    if name == 'lj3':
+        kw.setdefault('n', 2000)
+        return workloads.lj3(**kw)
+    if name == 'wall3':
+        kw.setdefault('n', 2000)
+        return workloads.wall3(**kw)
+    if name == 'patchy3':
+        kw.setdefault('n_mol', 400)
+        return workloads.patchy3(**kw)
     raise KeyError(name)
 
 
@@ -38,7 +47,7 @@ def engine_for(s, precision='fp64', rng='philox', seed=2001, graph_steps=0,
     eng = Engine([model], [rep], precision=precision, rng=rng, reaction_prob=reaction_prob,
                  bonds=s['bonds'], has_molecules=has_mol, graph_steps=graph_steps,
                  tile_particles=tile_particles, tile_rows=tile_rows, skin=skin,
-                 list_capacity=list_capacity, angles=s.get('angles'))
+                 list_capacity=list_capacity, angles=s.get('angles'), n_dim=len(s['L']))
     eng.set_state(s['x'], s['types'], s['molecule_ids'][:, 0] if has_mol else None)
     return eng
 
@@ -61,7 +70,7 @@ def oracle_for(s, rng='philox', seed=2001, nl='exact', **kw):
 
 def min_image(d, L, periodic):
     d = d.copy()
-    for k in range(2):
+    for k in range(len(L)):
         if periodic[k] > 0:
             d[..., k] -= L[k] * np.round(d[..., k] / L[k])
     return d

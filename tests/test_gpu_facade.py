@@ -19,8 +19,9 @@ def _write_inputs(d, s, n_steps, stride, n_out, extra=''):
     L = s['L']
     per = s['periodic']
     with open(os.path.join(d, 'parameters.txt'), 'w') as f:
-        f.write('N_dim = 2\nL = %r %r\nperiodic = %d %d\n'
-                % (float(L[0]), float(L[1]), int(per[0]), int(per[1])))
+        f.write('N_dim = %d\nL = %s\nperiodic = %s\n'
+                % (len(L), ' '.join(repr(float(v)) for v in L),
+                   ' '.join('%d' % int(v) for v in per)))
         f.write('dt = %r\nN_steps = %d\nstride = %d\nN_out = %d\n'
                 % (float(s['dt']), n_steps, stride, n_out))
         f.write('tau_by_tauD = %s\n' % ' '.join('1.0' for _ in s['model']['tau_by_tauD']))

@@ -124,3 +124,41 @@ def test_reference_configuration_atom_count_is_checked(tmp_path):
         s.create_initial_conditions({0: 3}, {0: [0, 1, 1, 1, 1]},
                                     reference_configurations={
                                         0: os.path.join(EX, 'reference_configuration_3.txt')})
+
+
+def test_random_rotations_3d_are_uniform_proper_rotations():
+    """Haar-random rotations: orthonormal with determinant +1, and the image
+    of a fixed axis is uniform on the sphere (mean ~ 0, second moment I/3)."""
+    rng = np.random.Generator(np.random.PCG64(3))
+    v = []
+    for _ in range(4000):
+        q = icg.random_rotation(3, rng)
+        assert np.abs(q @ q.T - np.eye(3)).max() < 1e-12
+        assert abs(np.linalg.det(q) - 1.0) < 1e-12
+        v.append(q @ np.array([0.0, 0.0, 1.0]))
+    v = np.array(v)
+    assert np.abs(v.mean(axis=0)).max() < 0.05
+    assert np.abs(v.T @ v / len(v) - np.eye(3) / 3).max() < 0.03
+
+
+@pytest.mark.parametrize('periodic', [[1, 1, 1], [1, -1, 1]])
+def test_rigid_placement_3d(periodic):
+    """Three-dimensional placement: rows (molecule, index, id, type, x, y, z),
+    rigid copies of a non-planar reference, inside the box, every pair of
+    molecules at least the threshold apart (minimum image where periodic)."""
+    ref = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    types = {0: [0, 1, 1, 1]}
+    L = np.array([12.0, 10.0, 11.0])
+    rng = np.random.Generator(np.random.PCG64(21))
+    out = icg.place_molecules({0: 40}, types, {0: ref}, L, periodic, 1.0, rng)
+    assert out.shape == (160, 7)
+    mol = out[:, 0].astype(int)
+    for m in range(40):
+        rows = out[mol == m]
+        assert np.abs(_pdist(rows[:, 4:]) - _pdist(ref)).max() < 1e-12
+        # proper rotation: the chirality of the reference is kept
+        e = rows[1:, 4:] - rows[0, 4:]
+        assert np.linalg.det(e) > 0
+    x = out[:, 4:]
+    assert (x >= 0).all() and (x <= L).all()
+    assert _brute_min(x, mol, L, periodic) >= 1.0

@@ -81,7 +81,7 @@ def _drift_brute(s):
             if (min(types[i], types[j]), max(types[i], types[j])) != (lo, hi) or r > it['rc']:
                 continue
             d = x[j] - x[i]
-            for k in range(2):
+            for k in range(len(L)):
                 if per[k] > 0:
                     d[k] -= L[k] * round(d[k] / L[k])
             f = force_law(it, r)
@@ -90,10 +90,10 @@ def _drift_brute(s):
     return a
 
 
-@pytest.mark.parametrize('name', ['lj1', 'kalj', 'wall'])
+@pytest.mark.parametrize('name', ['lj1', 'kalj', 'wall', 'lj3', 'wall3'])
 def test_drift_matches_brute_force(name):
     s = small(name, n=400) if name != 'lj1' else small('lj1', n=400)
-    if name == 'wall':
+    if name in ('wall', 'wall3'):
         s['types'][::3] = 1
     o = oracle_for(s)
     o.init()
@@ -102,7 +102,7 @@ def test_drift_matches_brute_force(name):
     assert np.abs(a - b).max() <= 1e-10 * (np.abs(b).max() + 1)
 
 
-@pytest.mark.parametrize('name', ['lj1', 'wall'])
+@pytest.mark.parametrize('name', ['lj1', 'wall', 'lj3', 'wall3'])
 def test_exact_neighbour_list_is_all_pairs_within_range(name):
     s = small(name, n=300)
     o = oracle_for(s)

@@ -155,7 +155,9 @@ def test_system_defaults_and_errors(tmp_path):
     with pytest.raises(RuntimeError, match='periodic'):
         make_replica(0, 10, [10.0, 10.0], [0.0, 1.0], 0.1, [1.0], [], 1)
     r = make_replica(0, 10, [10.0, 10.0], [2.0, -0.5], 0.1, [1.0], [], 1)
-    assert list(r.periodic) == [1, 0]
+    assert list(r.periodic)[:2] == [1, 0]
+    r = make_replica(0, 10, [10.0, 10.0, 8.0], [2.0, -0.5, 1.0], 0.1, [1.0], [], 1)
+    assert list(r.periodic) == [1, 0, 1] and list(r.L) == [10.0, 10.0, 8.0]
 
 
 def test_bond_and_angle_lists_of_molecules(tmp_path):
