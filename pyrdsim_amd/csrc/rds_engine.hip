@@ -345,6 +345,13 @@ __device__ __noinline__ void record_bounds(DStatus* st, unsigned long long key, 
   atomicExch(&st->err_lost, 1);
 }
 
+// q / d for 0 <= q < 2^20 and 1 <= d < 2^10 from inv_d = 1.0f / d: exact,
+// since (q + 1/2) / d lies at least 1 / (2d) from an integer and the float
+// product errs by less than that
+__device__ __forceinline__ int div_small(int q, float inv_d) {
+  return (int)(((float)q + 0.5f) * inv_d);
+}
+
 __device__ __forceinline__ int cell_coord(double x, double cl, int l) {
   double q = floor(x / cl);
   if (!(q >= 0.0)) return 0;
@@ -438,23 +445,38 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
   const int t = threadIdx.x, nthr = blockDim.x;
   const int NR = nrows + 2;
   const int lx = Rp.lx, ly = Rp.ly;
-  for (int q = t; q < NR * (W + 2); q += nthr) {
-    int r = q / (W + 2), k = q - r * (W + 2);
-    int y = cy0 + r - 1;
-    bool ok = true;
-    if (y < 0) { if (Rp.sy) y += ly; else ok = false; }
-    else if (y >= ly) { if (Rp.sy) y -= ly; else ok = false; }
-    int x = cx0 - 1 + k;
-    if (x < 0) { if (Rp.sx) x += lx; else ok = false; }
-    else if (x >= lx) { if (Rp.sx) x -= lx; else ok = false; }
-    int cs = 0, n = 0;
-    if (ok) {
-      int c = Rp.cbase + y * lx + x;
-      cs = cstart[c];
-      n = cstart[c + 1] - cs;
+  // the staged cells' starts, two per thread with both loads in flight
+  const int ncs = NR * (W + 2);
+  const float inv_w2 = 1.0f / (float)(W + 2);
+  for (int q0 = t; q0 < ncs; q0 += 2 * nthr) {
+    int c[2], a[2], e[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int q = q0 + u * nthr;
+      const int r = div_small(q, inv_w2), k = q - r * (W + 2);
+      int y = cy0 + r - 1;
+      bool ok = q < ncs;
+      if (y < 0) { if (Rp.sy) y += ly; else ok = false; }
+      else if (y >= ly) { if (Rp.sy) y -= ly; else ok = false; }
+      int x = cx0 - 1 + k;
+      if (x < 0) { if (Rp.sx) x += lx; else ok = false; }
+      else if (x >= lx) { if (Rp.sx) x -= lx; else ok = false; }
+      c[u] = ok ? Rp.cbase + y * lx + x : -1;
     }
-    sh.cs[r][k] = cs;
-    sh.off[r][k + 1] = n;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      a[u] = c[u] >= 0 ? cstart[c[u]] : 0;
+      e[u] = c[u] >= 0 ? cstart[c[u] + 1] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int q = q0 + u * nthr;
+      if (q < ncs) {
+        const int r = div_small(q, inv_w2), k = q - r * (W + 2);
+        sh.cs[r][k] = a[u];
+        sh.off[r][k + 1] = e[u] - a[u];
+      }
+    }
   }
   __syncthreads();
   {
@@ -524,10 +546,11 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
   // (counting sort), so that the lanes of one wave get home particles with
   // windows of similar length.
   const int nc = nrows * W;
+  const float inv_w = 1.0f / (float)W;
   for (int q = t; q < SBINS; q += nthr) sh.sbin[q] = 0;
   __syncthreads();
   auto stencil = [&](int c) {
-    const int q = c / W, k = c - q * W;
+    const int q = div_small(c, inv_w), k = c - q * W;
     int S = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) S += sh.off[q + d][k + 3] - sh.off[q + d][k];
@@ -539,7 +562,7 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
   __syncthreads();
   for (int c = t; c < nc; c += nthr) {
     const int pos = atomicAdd(&sh.sbin[SBINS - 1 - stencil(c)], 1);
-    const int q = c / W, k = c - q * W;
+    const int q = div_small(c, inv_w), k = c - q * W;
     sh.ckey[pos] = (q << 6) | k;
     sh.chp[pos] = sh.off[q + 1][k + 2] - sh.off[q + 1][k + 1];
   }
@@ -573,33 +596,57 @@ __device__ __forceinline__ double2 near_cell(double2 q, const DRep& Rp, int cell
 
 // Stages staged indices [p0, p0 + cap) into LDS.  scell (list mode): cells of
 // the last rebuild, whose nearest images are staged.  cpos: the stored
-// positions are float pairs; SPM: stage float pairs (1) or FP64 (0).
+// positions are float pairs; SPM: stage float pairs (1) or FP64 (0).  Each
+// thread takes STAGE_U staged indices at a time and issues all their loads
+// before the first LDS store, so that a tile's staging costs one round trip
+// to memory per STAGE_U x blockDim indices rather than one per row segment.
+#define STAGE_U 2
 template <int R, bool MOL, int SPM = 0>
 __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
                            const double2* __restrict__ pos, const int* __restrict__ type,
                            const int* __restrict__ mol, double2* s_pos, int* s_ty, int* s_mol,
                            const DRep& Rp, const int* __restrict__ scell, int cpos = 0) {
   const int t = threadIdx.x, nthr = blockDim.x;
-  const int p1 = p0 + cap;
-  for (int r = 0; r < NR; ++r) {
-    const double sy = sh.row_sy[r];
-    for (int g = 0; g < 3; ++g) {
-      const int n = sh.seg_n[r][g];
-      if (n == 0) continue;
-      const int so = sh.row_base[r] + sh.seg_off[r][g];
-      const int a = max(so, p0), b = min(so + n, p1);
-      const double sx = sh.seg_sx[r][g];
-      const int sl = sh.seg_slot[r][g] - so;
-      for (int p = a + t; p < b; p += nthr) {
-        double2 q = rd_pos(pos, sl + p, cpos);
-        if (scell) q = near_cell(q, Rp, scell[sl + p]);
-        q.x = q.x + sx;
-        q.y = q.y + sy;
-        if (SPM) reinterpret_cast<float4*>(s_pos)[p - p0] = df_split(q.x, q.y);
-        else s_pos[p - p0] = q;
-        s_ty[p - p0] = type[sl + p];
-        if (MOL) s_mol[p - p0] = mol[sl + p];
-      }
+  const int p1 = min(p0 + cap, sh.total);
+  for (int pb = p0 + t; pb < p1; pb += STAGE_U * nthr) {
+    int src[STAGE_U], rg[STAGE_U];
+#pragma unroll
+    for (int u = 0; u < STAGE_U; ++u) {
+      const int p = min(pb + u * nthr, p1 - 1);
+      // row: the last with row_base <= p; segment: the last non-empty one
+      // starting at or before p's offset in the row
+      int r = 0;
+#pragma unroll
+      for (int q = 1; q < R + 2; ++q) r += (q < NR && sh.row_base[q] <= p) ? 1 : 0;
+      const int loc = p - sh.row_base[r];
+      int g = 0;
+      if (sh.seg_n[r][1] > 0 && loc >= sh.seg_off[r][1]) g = 1;
+      if (sh.seg_n[r][2] > 0 && loc >= sh.seg_off[r][2]) g = 2;
+      src[u] = sh.seg_slot[r][g] + (loc - sh.seg_off[r][g]);
+      rg[u] = r * 3 + g;
+    }
+    double2 q[STAGE_U];
+    int ty[STAGE_U], mo[STAGE_U], sc[STAGE_U];
+#pragma unroll
+    for (int u = 0; u < STAGE_U; ++u) {
+      q[u] = rd_pos(pos, src[u], cpos);
+      ty[u] = type[src[u]];
+      mo[u] = MOL ? mol[src[u]] : 0;
+      sc[u] = scell ? scell[src[u]] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < STAGE_U; ++u) {
+      const int p = pb + u * nthr;
+      if (p >= p1) break;
+      double2 v = q[u];
+      if (scell) v = near_cell(v, Rp, sc[u]);
+      const int r = rg[u] / 3, g = rg[u] - 3 * r;
+      v.x = v.x + sh.seg_sx[r][g];
+      v.y = v.y + sh.row_sy[r];
+      if (SPM) reinterpret_cast<float4*>(s_pos)[p - p0] = df_split(v.x, v.y);
+      else s_pos[p - p0] = v;
+      s_ty[p - p0] = ty[u];
+      if (MOL) s_mol[p - p0] = mo[u];
     }
   }
 }
@@ -617,11 +664,14 @@ struct Home {
   int lo[3], hi[3];
 };
 
+// s_home (a resident tile staged in FP64 without list-mode images): xi is
+// read from the staged copy, which holds home cells unshifted.
 template <int R>
 __device__ __forceinline__ void home_windows(const TileSh<R>& sh, int wv, int b, int lane,
                                              int nrows, int W,
                                              const double2* __restrict__ pos, Home<R>& h,
-                                             double2& xi, int cpos = 0) {
+                                             double2& xi, int cpos = 0,
+                                             const double2* s_home = nullptr) {
   const int g = (b * R + wv) * 64 + lane;
   const int nc = nrows * W;
   h.valid = g < sh.chp[nc];
@@ -645,7 +695,7 @@ __device__ __forceinline__ void home_windows(const TileSh<R>& sh, int wv, int b,
     const int r = g - sh.chp[lo];
     h.s = sh.cs[hr][k + 1] + r;
     h.own = sh.row_base[hr] + sh.off[hr][k + 1] + r;
-    xi = rd_pos(pos, h.s, cpos);
+    xi = s_home ? s_home[h.own] : rd_pos(pos, h.s, cpos);
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -731,11 +781,27 @@ __device__ __forceinline__ Real pair_coef(const DTerm& t, Real r2) {
 
 // Wall force along its direction (pyrdsim.pyx:1134-1136, 1979-1986):
 // a[i, dir] += -u0 (1 - tanh^2((x - r0)/a)) / (2a)
+// The wall law f = u0 (1 - tanh^2 z) a_inv / 2, z = (x - r0) a_inv
+// (pyrdsim.pyx:1134-1136), returned as the drift -f.  1 - tanh^2 z is
+// sech^2 z = 4e / (1 + e)^2 with e = exp(-2|z|): one exponential in place of
+// tanh (several times its instruction count), and no cancellation where
+// tanh^2 z is close to 1.  Real = float: single-precision exponential and
+// reciprocal, as the pair forces of that precision.
+template <typename Real = double>
 __device__ __forceinline__ double wall_force(const DWall& w, double x) {
-  double z = (x - w.r0) * w.a_inv;
-  double t = tanh(z);
-  return -(w.u0 * (1.0 - t * t) * w.a_inv / 2.0);
+  if (sizeof(Real) == 4) {
+    const float z = (float)((x - w.r0) * w.a_inv);
+    const float e = expf(-2.0f * fabsf(z));
+    const float q = 1.0f + e;
+    const float s = 4.0f * e * __builtin_amdgcn_rcpf(q * q);
+    return -(w.u0 * (double)s * w.a_inv * 0.5);
+  }
+  const double z = (x - w.r0) * w.a_inv;
+  const double e = exp(-2.0 * fabs(z));
+  const double q = 1.0 + e;
+  return -(w.u0 * (4.0 * e / (q * q)) * w.a_inv / 2.0);
 }
+
 
 // ---------------------------------------------------------------------------
 // force (+ integrate + bin) kernel.  FAST: every type pair of every model has
@@ -839,6 +905,28 @@ void k_force(const StepArgs A, int mode, int pw) {
     }
     for (int q = t; q < M.nterms; q += blockDim.x) s_term[q] = M.terms[q];
   }
+  // FAST: the constants (c_lj, r0^2, cutoff^2) of the type pairs (a, b),
+  // a, b in {0, 1}, zero where the pair has no term, and the prefactors of
+  // types 0 and 1: one table per tile in LDS, read per lane by type
+  __shared__ Real s_fc[FAST ? 4 : 1][3];
+  __shared__ double s_fp[FAST ? 2 : 1][2];
+  if (FAST && t < 4) {
+    const int a = t >> 1, bb = t & 1, q = a * nt + bb;
+    Real v0 = 0, v1 = 0, v2 = 0;
+    if (a < nt && bb < nt && M.pcount[q] > 0) {
+      const DTerm& tm = M.terms[M.pfirst[q]];
+      v0 = TP<Real>::c_lj(tm);
+      v1 = TP<Real>::r0sq(tm);
+      v2 = TP<Real>::rc2(tm);
+    }
+    s_fc[t][0] = v0;
+    s_fc[t][1] = v1;
+    s_fc[t][2] = v2;
+    if (bb == 0) {
+      s_fp[a][0] = Rp.fpre[a];
+      s_fp[a][1] = Rp.npre[a];
+    }
+  }
   tile_setup<R>(Rp, A.cstart, cy0, cx0, W, nrows, sh);
   const int total = sh.total;
   const bool resident = total <= CAPT;
@@ -851,18 +939,27 @@ void k_force(const StepArgs A, int mode, int pw) {
   long long step_next = 0;
   if (mode == MODE_STEP) step_next = *A.step + 1;
 
+  // resident FP64 staging (not list mode): the home particles' position, type
+  // and molecule are read from LDS
+  const bool lds_home = resident && !LIST && PM == 0;
+
   for (int b = 0; b < sh.nbatch; ++b) {
     Home<R> h;
     double2 xi;
-    home_windows<R>(sh, wv, b, lane, nrows, W, A.pos, h, xi, PM);
+    home_windows<R>(sh, wv, b, lane, nrows, W, A.pos, h, xi, PM, lds_home ? s_pos : nullptr);
     int ti = 0, mi = 0, p = 0;
     // position in the frame of the staged tile (list mode: nearest image to
     // the sorted cell); xi itself stays the integrator's coordinate
     double2 xs = xi;
     if (h.valid) {
-      ti = A.type[h.s];
       p = A.pid[h.s];
-      if (MOL) mi = A.mol[h.s];
+      if (lds_home) {
+        ti = s_ty[h.own];
+        if (MOL) mi = s_mol[h.own];
+      } else {
+        ti = A.type[h.s];
+        if (MOL) mi = A.mol[h.s];
+      }
       if (LIST) xs = near_cell(xi, Rp, scell[h.s]);
     }
     // list mode: the lane walks its partner list; a lane whose list overflowed
@@ -881,15 +978,13 @@ void k_force(const StepArgs A, int mode, int pw) {
     // no term for the pair)
     Real pa0 = 0, pb0 = 0, pc0 = 0, pa1 = 0, pb1 = 0, pc1 = 0;
     if (FAST) {
-      int q0 = ti * nt;
-      if (M.pcount[q0] > 0) {
-        const DTerm& tm = M.terms[M.pfirst[q0]];
-        pa0 = TP<Real>::c_lj(tm); pb0 = TP<Real>::r0sq(tm); pc0 = TP<Real>::rc2(tm);
-      }
-      if (nt > 1 && M.pcount[q0 + 1] > 0) {
-        const DTerm& tm = M.terms[M.pfirst[q0 + 1]];
-        pa1 = TP<Real>::c_lj(tm); pb1 = TP<Real>::r0sq(tm); pc1 = TP<Real>::rc2(tm);
-      }
+      const int a = ti == 1 ? 2 : 0;
+      pa0 = s_fc[a][0];
+      pb0 = s_fc[a][1];
+      pc0 = s_fc[a][2];
+      pa1 = s_fc[a + 1][0];
+      pb1 = s_fc[a + 1][1];
+      pc1 = s_fc[a + 1][2];
     }
     Real fx = 0, fy = 0;
     const int tbase = ti * nt;
@@ -1003,8 +1098,8 @@ void k_force(const StepArgs A, int mode, int pw) {
     for (int q = 0; q < M.nwalls; ++q) {
       const DWall& wl = M.walls[q];
       if (wl.type != ti) continue;
-      if (wl.dir == 0) Fx += wall_force(wl, xi.x);
-      else Fy += wall_force(wl, xi.y);
+      if (wl.dir == 0) Fx += wall_force<Real>(wl, xi.x);
+      else Fy += wall_force<Real>(wl, xi.y);
     }
     if (A.fbond) {
       double2 fb = A.fbond[s];
@@ -1067,8 +1162,8 @@ void k_force(const StepArgs A, int mode, int pw) {
       }
       reinterpret_cast<float4*>(A.upos)[s] = make_float4(u2[0].h, u2[1].h, u2[0].l, u2[1].l);
     } else {
-      const double fp = Rp.fpre[ti];
-      const double np = A.include_rf * Rp.npre[ti];
+      const double fp = FAST ? s_fp[ti == 1][0] : Rp.fpre[ti];
+      const double np = A.include_rf * (FAST ? s_fp[ti == 1][1] : Rp.npre[ti]);
       double x = xi.x, y = xi.y;
       {
 #pragma clang fp contract(off)
@@ -2282,7 +2377,7 @@ __global__ __launch_bounds__(256) void k3_force(const StepArgs A, int mode) {
   const double X[3] = {xi.x, xi.y, xi.z};
   for (int q = 0; q < M.nwalls; ++q) {
     const DWall& wl = M.walls[q];
-    if (wl.type == ti) F[wl.dir] += wall_force(wl, X[wl.dir]);
+    if (wl.type == ti) F[wl.dir] += wall_force<Real>(wl, X[wl.dir]);
   }
   if (A.fbond4) {
     const double4 fb = A.fbond4[s];
