@@ -345,6 +345,9 @@ def main():
     ap.add_argument('--state', default=None,
                     help='.npz file (x, t, step) to start the batched run from instead of the '
                          'initial states (the counter passes use it)')
+    ap.add_argument('--save-state', default=None,
+                    help='.npz file to write the batched state to after the timed window, so '
+                         'that a profiled run can start from it with --state')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-autotune', dest='autotune', action='store_false')
     ap.add_argument('--no-accuracy', action='store_true')
@@ -427,6 +430,8 @@ def main():
                       graph_steps=graph_steps)
         print('# steady state: %s' % steady, file=sys.stderr, flush=True)
 
+    if args.save_state:
+        save_state(eng, args.save_state)
     kt = eng.time_kernels(20)
     eng.sync()
     st2 = eng.sync()

@@ -37,3 +37,28 @@ def test_activated_fraction_and_clusters():
     # chain of 3, a pair, and one isolated (19.8, 5) whose periodic image is near
     # (0.2, 5), which is inactive
     assert sorted(sizes.tolist()) == [1, 2, 3]
+
+
+def test_ideal_gas_rdf_is_one_in_three_dimensions():
+    rng = np.random.Generator(np.random.PCG64(1))
+    L = np.array([40.0, 40.0, 40.0])
+    x = rng.random((40000, 3)) * L
+    r, g = analysis.rdf(x, L, [1, 1, 1], r_max=4.0, dr=0.2)
+    assert np.abs(g[3:] - 1).max() < 0.05
+    # the same normalisation from a pair histogram
+    counts = np.round(g * 40000 * (40000 / L.prod()) * analysis._shell(
+        np.arange(0.0, 4.0 + 0.1, 0.2), 3) / 2.0)
+    _, g2 = analysis.rdf_from_counts(counts, 40000, L, 4.0, 0.2, n_dim=3)
+    assert np.allclose(g2, g, rtol=1e-6, atol=1e-6)
+
+
+def test_two_dimensional_points_with_three_entry_box():
+    # replica records carry three box lengths; the coordinates' columns decide
+    rng = np.random.Generator(np.random.PCG64(2))
+    x = rng.random((20000, 2)) * 100.0
+    _, g = analysis.rdf(x, [100.0, 100.0, 0.0], [1, 1, 0], r_max=3.0, dr=0.1)
+    _, g2 = analysis.rdf(x, [100.0, 100.0], [1, 1], r_max=3.0, dr=0.1)
+    assert np.array_equal(g, g2)
+    sizes = analysis.cluster_sizes(x, np.ones(len(x), int), [100.0, 100.0, 0.0], [1, 1, 0],
+                                   r_link=0.3)
+    assert sizes.sum() == len(x)
