@@ -51,7 +51,7 @@
 #define MAXWALLS RDS_MAX_WALLS
 #define MAXRX RDS_MAX_REACTIONS
 #define MAXW 60        // cells per tile strip
-#define SBINS 128      // bins of the per-tile sort of home cells by stencil count
+#define HMAX 1024      // home particles per tile with a direct home-number -> cell table
 
 namespace {
 
@@ -405,9 +405,8 @@ struct TileSh {
   double seg_sx[R + 2][3];
   double row_sy[R + 2];
   int hp[R + 1];             // home particles of the home rows before row q
-  int ckey[R * MAXW];        // home cells (q << 6 | k), by stencil count, largest first
-  int chp[R * MAXW + 1];     // home particles of the sorted cells before cell i
-  int sbin[SBINS];
+  int chp[R * MAXW + 1];     // home particles of the home cells before cell i = q * W + k
+  unsigned char hkey[HMAX];  // home number -> its cell (q << 6 | k), when <= HMAX of them
   int total, nbatch;
 };
 
@@ -542,29 +541,13 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
     nb = (hsum + 64 * R - 1) / (64 * R);
     sh.nbatch = nb;
   }
-  // Home cells in order of their stencil's candidate count, largest first
-  // (counting sort), so that the lanes of one wave get home particles with
-  // windows of similar length.
+  // Home cells in row-major order (cell i = q * W + k): the home particles
+  // before each cell, and for each home number its cell.
   const int nc = nrows * W;
   const float inv_w = 1.0f / (float)W;
-  for (int q = t; q < SBINS; q += nthr) sh.sbin[q] = 0;
-  __syncthreads();
-  auto stencil = [&](int c) {
-    const int q = div_small(c, inv_w), k = c - q * W;
-    int S = 0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) S += sh.off[q + d][k + 3] - sh.off[q + d][k];
-    return min(S, SBINS - 1);
-  };
-  for (int c = t; c < nc; c += nthr) atomicAdd(&sh.sbin[SBINS - 1 - stencil(c)], 1);
-  __syncthreads();
-  if (t < 64) wave_exclusive_scan<SBINS / 64>(sh.sbin, SBINS, t);
-  __syncthreads();
   for (int c = t; c < nc; c += nthr) {
-    const int pos = atomicAdd(&sh.sbin[SBINS - 1 - stencil(c)], 1);
     const int q = div_small(c, inv_w), k = c - q * W;
-    sh.ckey[pos] = (q << 6) | k;
-    sh.chp[pos] = sh.off[q + 1][k + 2] - sh.off[q + 1][k + 1];
+    sh.chp[c] = sh.off[q + 1][k + 2] - sh.off[q + 1][k + 1];
   }
   __syncthreads();
   if (t < 64) {
@@ -572,6 +555,14 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
     if (t == 0) sh.chp[nc] = tot;
   }
   __syncthreads();
+  if (sh.chp[nc] <= HMAX) {
+    for (int c = t; c < nc; c += nthr) {
+      const int q = div_small(c, inv_w), k = c - q * W;
+      const unsigned char key = (unsigned char)((q << 6) | k);
+      for (int g = sh.chp[c]; g < sh.chp[c + 1]; ++g) sh.hkey[g] = key;
+    }
+    __syncthreads();
+  }
 }
 
 // List mode: a periodic coordinate may have wrapped since the slot was sorted
@@ -652,7 +643,7 @@ __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
 }
 
 // Home particle of lane `lane` of wave wv in batch b, and its three windows.
-// The tile's home particles, taken cell by cell in the order of the sorted
+// The tile's home particles, taken cell by cell in row-major order of the
 // home cells, are dealt to the waves 64 at a time: home number
 // (b * R + wv) * 64 + lane.  The home cell is the cell whose slot range holds
 // the particle (the cell it was sorted into).
@@ -674,18 +665,28 @@ __device__ __forceinline__ void home_windows(const TileSh<R>& sh, int wv, int b,
                                              const double2* s_home = nullptr) {
   const int g = (b * R + wv) * 64 + lane;
   const int nc = nrows * W;
-  h.valid = g < sh.chp[nc];
-  // the sorted cell holding home number g: largest i with chp[i] <= g
-  int lo = 0, hi = nc;
-  if (h.valid) {
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (sh.chp[mid] <= g) lo = mid;
-      else hi = mid;
+  const int nh = sh.chp[nc];
+  h.valid = g < nh;
+  // the cell holding home number g: from the table, or the largest i with
+  // chp[i] <= g
+  int q = 0, k = 0, lo = 0;
+  if (nh <= HMAX) {
+    const int key = h.valid ? sh.hkey[g] : 0;
+    q = key >> 6;
+    k = key & 63;
+    lo = q * W + k;
+  } else {
+    int hi = nc;
+    if (h.valid) {
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (sh.chp[mid] <= g) lo = mid;
+        else hi = mid;
+      }
     }
+    q = div_small(lo, 1.0f / (float)W);
+    k = lo - q * W;
   }
-  const int key = sh.ckey[lo];
-  const int q = key >> 6, k = key & 63;
   const int hr = q + 1;  // staged row of the home row
   xi = make_double2(0.0, 0.0);
   h.s = 0;
