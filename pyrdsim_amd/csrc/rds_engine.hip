@@ -1580,10 +1580,13 @@ __global__ __launch_bounds__(256) void k_react(const StepArgs A) {
 // inhibited reaction), one wave per slot block, one lane per listed slot.
 // The block's replica is wave-uniform (its tables come through scalar
 // loads), and the parameters of the slot's neighbour-dependent reactions
-// are taken into registers before its candidate loop; each stencil row is
-// at most two contiguous slot ranges, walked four candidates at a time.
+// are taken into registers before its candidate loop.  The cell starts of
+// the three stencil rows are loaded together; each row is at most two
+// contiguous slot ranges, whose candidates' types are loaded WALK_BATCH at a
+// time (positions only for the types a reaction counts).
+#define WALK_BATCH 12
 template <bool MOL>
-__global__ __launch_bounds__(256) void k_react_walk(const StepArgs A, int n_blocks) {
+__global__ __launch_bounds__(256, 6) void k_react_walk(const StepArgs A, int n_blocks) {
   const int sb = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (sb >= n_blocks) return;
   const int n = A.wl_n[sb];
@@ -1627,47 +1630,63 @@ __global__ __launch_bounds__(256) void k_react_walk(const StepArgs A, int n_bloc
     unsigned long long c_lo = 0ull, c_hi = 0ull;  // packed counts (cnt_of)
     const int cx = cell_coord(xi.x, cl0, lx);
     const int cy = cell_coord(xi.y, cl1, ly);
-#pragma unroll 1
-    for (int oy = -1; oy <= 1; ++oy) {
-      int y = cy + oy;
+    // the stencil rows' cells cx-1 .. cx+1, each as at most two contiguous
+    // slot ranges [ra0, ra1) and [rb0, rb1); a row outside a reflecting wall
+    // is empty.  The cell-start loads of all three rows are issued together.
+    int ra0[3], ra1[3], rb0[3], rb1[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      int y = cy + r - 1;
+      bool ok = true;
       if (y < 0 || y >= ly) {
-        if (!sy) continue;
+        ok = sy != 0;
         y = (y < 0) ? y + ly : y - ly;
       }
-      // cells cx-1 .. cx+1 of the row as at most two contiguous slot ranges
-      const int rowc = cbase + y * lx;
-      int a0, a1, b0r = 0, b1r = 0;
+      const int rowc = cbase + (ok ? y : 0) * lx;
+      int i0, i1, i2 = rowc, i3 = rowc;  // cstart indices
       if (!sx) {
-        a0 = A.cstart[rowc + max(cx - 1, 0)];
-        a1 = A.cstart[rowc + min(cx + 1, lx - 1) + 1];
+        i0 = rowc + max(cx - 1, 0);
+        i1 = rowc + min(cx + 1, lx - 1) + 1;
       } else if (cx == 0) {
-        a0 = A.cstart[rowc + lx - 1];
-        a1 = A.cstart[rowc + lx];
-        b0r = A.cstart[rowc];
-        b1r = A.cstart[rowc + 2];
+        i0 = rowc + lx - 1;
+        i1 = rowc + lx;
+        i3 = rowc + 2;
       } else if (cx == lx - 1) {
-        a0 = A.cstart[rowc + cx - 1];
-        a1 = A.cstart[rowc + lx];
-        b0r = A.cstart[rowc];
-        b1r = A.cstart[rowc + 1];
+        i0 = rowc + cx - 1;
+        i1 = rowc + lx;
+        i3 = rowc + 1;
       } else {
-        a0 = A.cstart[rowc + cx - 1];
-        a1 = A.cstart[rowc + cx + 2];
+        i0 = rowc + cx - 1;
+        i1 = rowc + cx + 2;
       }
-      const int na = a1 - a0, nn = na + (b1r - b0r);
+      ra0[r] = A.cstart[i0];
+      ra1[r] = ok ? A.cstart[i1] : ra0[r];
+      rb0[r] = A.cstart[i2];
+      rb1[r] = ok ? A.cstart[i3] : rb0[r];
+    }
 #pragma unroll 1
-      for (int u = 0; u < nn; u += 4) {
-        int jj[4], tt[4];
+    for (int r = 0; r < 3; ++r) {
+      const int a0 = ra0[r], b0r = rb0[r];
+      const int na = ra1[r] - a0, nn = na + (rb1[r] - b0r);
+#pragma unroll 1
+      for (int u = 0; u < nn; u += WALK_BATCH) {
+        // the batch's candidates of a counted type (other than the slot), as
+        // a bit mask: the type loads of the batch are in flight together
+        unsigned hit = 0u;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < WALK_BATCH; ++v) {
           const int w = u + v;
-          jj[v] = w < na ? a0 + w : b0r + (w - na);
-          tt[v] = w < nn ? A.type[jj[v]] : -1;
+          const int j = w < na ? a0 + w : b0r + (w - na);
+          const int tj = w < nn ? A.type[j] : -1;
+          if (j != s && tj >= 0 && tj < 32 && ((nbr_types >> tj) & 1u)) hit |= 1u << v;
         }
 #pragma unroll 1
-        for (int v = 0; v < 4; ++v) {
-          const int j = jj[v], tj = tt[v];
-          if (j == s || tj < 0 || tj >= 32 || !((nbr_types >> tj) & 1u)) continue;
+        while (hit) {
+          const int v = __ffs(hit) - 1;
+          hit &= hit - 1u;
+          const int w = u + v;
+          const int j = w < na ? a0 + w : b0r + (w - na);
+          const int tj = A.type[j];
           // the reference's neighbour list holds other molecules' particles
           // and the bonded partners (pyrdsim.pyx:1513, 1523)
           if (MOL && A.mol[j] == mi) {
