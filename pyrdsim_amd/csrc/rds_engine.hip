@@ -41,6 +41,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -3333,7 +3334,14 @@ bool build_tiles(rds_ctx* c, int rows, int target) {
   c->rows = rows;
   c->h_tiles.clear();
   double staged_worst = 0.0;
-  for (int r = 0; r < c->n_rep; ++r) {
+  // The force kernel gives each XCD a contiguous eighth of the tile list.
+  // Replicas enter the list in a stride order (stride about 0.618 n_rep,
+  // coprime with it), so that every eighth mixes replicas of all models and
+  // sizes; a replica's own tiles stay together (halo rows meet in one L2).
+  int stride = std::max(1, (int)std::lround(0.618 * c->n_rep));
+  while (std::gcd(stride, c->n_rep) != 1) ++stride;
+  for (int k = 0; k < c->n_rep; ++k) {
+    const int r = (int)(((long long)k * stride) % c->n_rep);
     const DRep& d = c->h_rep[r];
     double occ = (double)d.n / ((double)d.lx * d.ly);
     int W = (int)std::floor((double)target / std::max(occ, 1e-9));
