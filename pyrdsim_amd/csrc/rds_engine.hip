@@ -2043,21 +2043,32 @@ __global__ __launch_bounds__(CHUNK_THREADS) void k_cellscan(
     DStatus* status) {
   __shared__ int s_red[4];
   __shared__ int s_wsum[CHUNK_THREADS / 64];
+  __shared__ __align__(16) int s_v[CHUNK_CELLS];
   const int t = threadIdx.x, b = blockIdx.x;
   const int4 ch = chunks[b];
   const DRep& R = rep[ch.x];
   const int c0 = R.cbase + ch.y;
-  // this thread's cells first, then the particles in the replica's chunks
-  // before this one: the two sets of loads are in flight together
+  // the chunk's histogram through LDS, so that global loads and stores are
+  // coalesced while each thread scans PER consecutive cells; the particles
+  // in the replica's chunks before this one are loaded in the same round
   constexpr int PER = CHUNK_CELLS / CHUNK_THREADS;
-  int v[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int c = t * PER + i;
-    v[i] = c < ch.z ? count[c0 + c] : 0;
+    const int c = i * CHUNK_THREADS + t;
+    s_v[c] = c < ch.z ? count[c0 + c] : 0;
   }
   int pre = 0;
   for (int k = ch.w + t; k < b; k += CHUNK_THREADS) pre += csum[k];
+  __syncthreads();
+  int v[PER];
+#pragma unroll
+  for (int i = 0; i < PER; i += 4) {
+    const int4 q = *reinterpret_cast<const int4*>(&s_v[t * PER + i]);
+    v[i] = q.x;
+    v[i + 1] = q.y;
+    v[i + 2] = q.z;
+    v[i + 3] = q.w;
+  }
   int tsum = 0;
 #pragma unroll
   for (int i = 0; i < PER; ++i) tsum += v[i];
@@ -2075,12 +2086,21 @@ __global__ __launch_bounds__(CHUNK_THREADS) void k_cellscan(
   int excl = base + wbase + inc - tsum;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int c = t * PER + i;
+    const int e = excl;
+    excl += v[i];
+    v[i] = e;
+  }
+#pragma unroll
+  for (int i = 0; i < PER; i += 4)
+    *reinterpret_cast<int4*>(&s_v[t * PER + i]) = make_int4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int c = i * CHUNK_THREADS + t;
     if (c < ch.z) {
-      cstart[c0 + c] = excl;
+      cstart[c0 + c] = s_v[c];
       count[c0 + c] = 0;
     }
-    excl += v[i];
   }
   // the replica's last chunk checks the total
   if (t == CHUNK_THREADS - 1 && ch.y + ch.z == R.ncell && excl != R.pbase + R.n)
