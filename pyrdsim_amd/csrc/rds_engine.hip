@@ -524,23 +524,25 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
       }
     }
   }
-  if (t == 0) {
-    int base = 0, nb = 0;
-    for (int r = 0; r < NR; ++r) {
-      sh.row_base[r] = base;
-      base += sh.off[r][W + 2];
+  if (t < 64) {
+    // first staged index of each staged row and home particles before each
+    // home row: prefix sums over the lanes (NR <= R + 2 <= 6 rows)
+    const int rt = t < NR ? sh.off[t][W + 2] : 0;
+    const int ht = t < nrows ? sh.off[t + 1][W + 1] - sh.off[t + 1][1] : 0;
+    int ir = rt, ih = ht;
+#pragma unroll
+    for (int d = 1; d < 8; d <<= 1) {
+      const int a = __shfl_up(ir, d, 64), e = __shfl_up(ih, d, 64);
+      if (t >= d) {
+        ir += a;
+        ih += e;
+      }
     }
-    sh.row_base[NR] = base;
-    sh.total = base;
-    int hsum = 0;
-    for (int q = 0; q < nrows; ++q) {
-      sh.hp[q] = hsum;
-      hsum += sh.off[q + 1][W + 1] - sh.off[q + 1][1];
-    }
-    sh.hp[nrows] = hsum;
+    if (t <= NR) sh.row_base[t] = ir - rt;
+    if (t == NR) sh.total = ir;
+    if (t <= nrows) sh.hp[t] = ih - ht;
     // the home particles of all rows are dealt to the R waves, 64 per wave
-    nb = (hsum + 64 * R - 1) / (64 * R);
-    sh.nbatch = nb;
+    if (t == nrows) sh.nbatch = (ih + 64 * R - 1) / (64 * R);
   }
   // Home cells in row-major order (cell i = q * W + k): the home particles
   // before each cell, and for each home number its cell.
@@ -586,14 +588,14 @@ __device__ __forceinline__ double2 near_cell(double2 q, const DRep& Rp, int cell
   return q;
 }
 
-// Stages staged indices [p0, p0 + cap) into LDS.  scell (list mode): cells of
+// Stages staged indices [p0, p0 + cap) into LDS (types shifted left by TSH).  scell (list mode): cells of
 // the last rebuild, whose nearest images are staged.  cpos: the stored
 // positions are float pairs; SPM: stage float pairs (1) or FP64 (0).  Each
 // thread takes STAGE_U staged indices at a time and issues all their loads
 // before the first LDS store, so that a tile's staging costs one round trip
 // to memory per STAGE_U x blockDim indices rather than one per row segment.
 #define STAGE_U 2
-template <int R, bool MOL, int SPM = 0>
+template <int R, bool MOL, int SPM = 0, int TSH = 0>
 __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
                            const double2* __restrict__ pos, const int* __restrict__ type,
                            const int* __restrict__ mol, double2* s_pos, int* s_ty, int* s_mol,
@@ -637,7 +639,7 @@ __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
       v.y = v.y + sh.row_sy[r];
       if (SPM) reinterpret_cast<float4*>(s_pos)[p - p0] = df_split(v.x, v.y);
       else s_pos[p - p0] = v;
-      s_ty[p - p0] = ty[u];
+      s_ty[p - p0] = ty[u] << TSH;
       if (MOL) s_mol[p - p0] = mo[u];
     }
   }
@@ -912,6 +914,11 @@ void k_force(const StepArgs A, int mode, int pw) {
   // types 0 and 1: one table per tile in LDS, read per lane by type
   __shared__ Real s_fc[FAST ? 4 : 1][3];
   __shared__ double s_fp[FAST ? 2 : 1][2];
+  // FP32 pair arithmetic: the same constants as float4 entries (a, b) at
+  // byte 32 a + 16 b, the staged types holding 16 b, so that a candidate's
+  // constants are one LDS read at its lane's row plus its staged type
+  constexpr int TSH = (FAST && sizeof(Real) == 4) ? 4 : 0;
+  __shared__ __align__(16) float4 s_fc4[TSH ? 4 : 1];
   if (FAST && t < 4) {
     const int a = t >> 1, bb = t & 1, q = a * nt + bb;
     Real v0 = 0, v1 = 0, v2 = 0;
@@ -924,6 +931,7 @@ void k_force(const StepArgs A, int mode, int pw) {
     s_fc[t][0] = v0;
     s_fc[t][1] = v1;
     s_fc[t][2] = v2;
+    if (TSH) s_fc4[t] = make_float4((float)v0, (float)v1, (float)v2, 0.f);
     if (bb == 0) {
       s_fp[a][0] = Rp.fpre[a];
       s_fp[a][1] = Rp.npre[a];
@@ -934,7 +942,7 @@ void k_force(const StepArgs A, int mode, int pw) {
   const bool resident = total <= CAPT;
   const int* scell = LIST ? A.scell : nullptr;
   if (resident) {
-    tile_stage<R, MOL, PM>(sh, NR, 0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, scell,
+    tile_stage<R, MOL, PM, TSH>(sh, NR, 0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, scell,
                            PM);
     __syncthreads();
   }
@@ -956,7 +964,7 @@ void k_force(const StepArgs A, int mode, int pw) {
     if (h.valid) {
       p = A.pid[h.s];
       if (lds_home) {
-        ti = s_ty[h.own];
+        ti = s_ty[h.own] >> TSH;
         if (MOL) mi = s_mol[h.own];
       } else {
         ti = A.type[h.s];
@@ -995,12 +1003,24 @@ void k_force(const StepArgs A, int mode, int pw) {
     // filtered here: self / same molecule never interact, and a particle's only
     // zero-distance partner in an atomic model is itself.
     const float4 xsf = PM ? df_split(xs.x, xs.y) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const char* fc_row = reinterpret_cast<const char*>(s_fc4) + ti * 32;
     auto pairv = [&](const Real dx, const Real dy, int tj, int mj, bool live, bool listed) {
       const Real r2 = dx * dx + dy * dy;
       const bool other = (MOL && listed) ? true : (MOL ? (mj != mi) : (r2 > Real(0)));
       if (FAST) {
-        const bool m1 = tj == 1;
-        const Real pa = m1 ? pa1 : pa0, pb = m1 ? pb1 : pb0, pc = m1 ? pc1 : pc0;
+        Real pa, pb, pc;
+        if (TSH) {
+          // tj = 16 * type: the (ti, tj) constants from the tile's table
+          const float4 kc = *reinterpret_cast<const float4*>(fc_row + tj);
+          pa = (Real)kc.x;
+          pb = (Real)kc.y;
+          pc = (Real)kc.z;
+        } else {
+          const bool m1 = tj == 1;
+          pa = m1 ? pa1 : pa0;
+          pb = m1 ? pb1 : pb0;
+          pc = m1 ? pc1 : pc0;
+        }
         Real c = lj_fast<Real, KM>(r2, pa, pb, pw);
         // atomic models: r2 > 0 excludes the particle itself, inside in_range
         const bool take = MOL ? (other && r2 <= pc) : in_range(r2, pc);
@@ -1038,7 +1058,7 @@ void k_force(const StepArgs A, int mode, int pw) {
     for (int p0 = 0; p0 < total; p0 += CAPT) {
       if (!resident) {
         __syncthreads();
-        tile_stage<R, MOL, PM>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp,
+        tile_stage<R, MOL, PM, TSH>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp,
                                scell, PM);
         __syncthreads();
       }
