@@ -38,6 +38,18 @@ def test_forces_match_oracle(gpu, name, precision, rtol):
     bad = (err > rtol * (scale + extra)) & ~near_cutoff(s)
     assert not bad.any(), (name, precision, int(bad.sum()),
                            float((err / (scale + extra + 1e-300))[bad].max()))
+    # plain relative errors: norm-wise over the system, and per particle over
+    # the particles whose force is at least the median magnitude (away from
+    # near-cancellations of their terms)
+    far = ~near_cutoff(s)
+    normwise = float(np.linalg.norm((f - a)[far]) / np.linalg.norm(a[far]))
+    mag = np.linalg.norm(a, axis=1)
+    solid = far & (mag >= np.median(mag))
+    rel = np.linalg.norm(f - a, axis=1)[solid] / mag[solid]
+    print('\nforces %s %s: ||dF||/||F|| = %.2e, max |dF_i|/|F_i| = %.2e over the %d of %d '
+          'particles with |F_i| >= median' % (name, precision, normwise, float(rel.max()),
+                                               int(solid.sum()), len(a)))
+    assert normwise <= rtol, (name, precision, normwise)
 
 
 @pytest.mark.parametrize('name', ['lj1', 'wall'])
