@@ -188,7 +188,7 @@ def measure_traffic(cache, state, rows, per, replicas, timeout=300):
     child = [sys.executable, os.path.abspath(__file__), '--steps', '6', '--warmup', '0',
              '--graph-steps', '0', '--pre-advance', '0', '--replicas', str(replicas),
              '--no-plain', '--no-cpu', '--no-accuracy', '--no-autotune', '--no-traffic',
-             '--no-analysis', '--no-c1', '--no-steady-state', '--tile-rows', str(rows),
+             '--no-analysis', '--no-c1', '--no-3d', '--no-steady-state', '--tile-rows', str(rows),
              '--tile-particles', str(per), '--copy-calibration', '--ic-cache', cache,
              '--state', state]
     out = {}
@@ -288,6 +288,28 @@ def c1_leg(runs=3, steps=2000):
                 unit='particle-steps/s')
 
 
+def three_d_leg(n=262144, steps=200, warmup=50):
+    """The three-dimensional engine (n_dim = 3, 27-cell stencil): the
+    two-species LJ mixture of the ensemble in a periodic cube at density 0.6,
+    N = 262,144, graphs of 50 steps, in mixed precision and in FP64, in
+    particle-steps/s."""
+    import torch
+    from pyrdsim_amd.engine import Engine, build_model, make_replica
+    s = workloads.lj3(n=n)
+    m = s['model']
+    out = dict(workload='two-species LJ mixture, 3-D periodic cube, density 0.6, N=%d, '
+                        '%d steps after %d' % (n, steps, warmup), unit='particle-steps/s')
+    for prec in ('mixed', 'fp64'):
+        model, rates = build_model(m['interactions'], m['reactions'], len(m['tau_by_tauD']))
+        rep = make_replica(0, n, s['L'], s['periodic'], s['dt'], m['tau_by_tauD'], rates, 2001)
+        e = Engine([model], [rep], precision=prec, graph_steps=50, n_dim=3)
+        e.set_state(s['x'], s['types'])
+        out[prec] = n * steps / time_engine(e, steps, warmup)
+        e.close()
+    torch.cuda.synchronize()
+    return out
+
+
 def copy_bandwidth(nbytes=2 << 30, reps=10):
     """Streaming copy of nbytes between two device buffers (torch's copy
     kernel), timed with HIP events: (read + write bytes) / time in GB/s, the
@@ -358,6 +380,7 @@ def main():
                          'written) first, to scale HBM counters in a profile of this command')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-c1', action='store_true')
+    ap.add_argument('--no-3d', dest='three_d', action='store_false')
     ap.add_argument('--no-analysis', action='store_true')
     ap.add_argument('--no-steady-state', action='store_true')
     ap.add_argument('--no-traffic', dest='traffic', action='store_false',
@@ -549,6 +572,10 @@ def main():
     if not args.no_c1:
         c1 = c1_leg()
         print('# c1: %s' % c1, file=sys.stderr, flush=True)
+    three_d = None
+    if args.three_d:
+        three_d = three_d_leg()
+        print('# 3-D: %s' % three_d, file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         v, desc = cpu_baseline(plan, states, args.cpu_seconds)
@@ -588,6 +615,7 @@ def main():
         'compensated_positions': compensated,
         'batched_fp64': batched_fp64,
         'analysis': ana,
+        'three_d': three_d,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))
