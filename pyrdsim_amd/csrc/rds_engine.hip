@@ -2402,38 +2402,127 @@ __device__ __forceinline__ double3 disp3(const DRep& R, double4 a, double4 b) {
 // Pair forces (every term of the type pair, file order), walls and bonds of
 // the particle in slot s; then, in MODE_STEP, the Euler-Maruyama update
 // (pyrdsim.pyx:2244-2250), the boundary conditions (pyrdsim.pyx:2269-2292)
-// and the binning of the new position.
-template <typename Real, bool MOL>
-__global__ __launch_bounds__(256) void k3_force(const StepArgs A, int mode) {
+// and the binning of the new position.  The 27-cell stencil is walked as nine
+// rows of three x-cells; a row is at most two contiguous slot ranges, each
+// with one periodic image shift (x_j + shift - x_i needs no minimum image:
+// a periodic axis with a stencil has at least three cells).  Candidates are
+// loaded four (general term table: two) at a time.  KM as in k_force: 0 = the general term table;
+// 1, 2 = one LJ term per type pair of at most two types, its constants in
+// registers (2: exponent n = 6).
+template <typename Real, bool MOL, int KM>
+__global__ __launch_bounds__(256) void k3_force(const StepArgs A, int mode, int pw) {
+  constexpr bool FAST = KM != 0;
   const int4 B = A.sblocks[blockIdx.x];
   const DRep& Rp = A.rep[B.x];
   const DModel& M = A.models[Rp.model];
+  __shared__ Real s_fc[FAST ? 4 : 1][3];
+  if (FAST) {
+    if (threadIdx.x < 4) {
+      const int a = threadIdx.x >> 1, bb = threadIdx.x & 1, q = a * M.nt + bb;
+      Real v0 = 0, v1 = 0, v2 = 0;
+      if (a < M.nt && bb < M.nt && M.pcount[q] > 0) {
+        const DTerm& tm = M.terms[M.pfirst[q]];
+        v0 = TP<Real>::c_lj(tm);
+        v1 = TP<Real>::r0sq(tm);
+        v2 = TP<Real>::rc2(tm);
+      }
+      s_fc[threadIdx.x][0] = v0;
+      s_fc[threadIdx.x][1] = v1;
+      s_fc[threadIdx.x][2] = v2;
+    }
+    __syncthreads();
+  }
   const int s = B.y + (int)threadIdx.x;
   if (s >= B.z) return;
   const double4 xi = A.pos4[s];
   const int ti = A.type[s], p = A.pid[s];
   const int mi = MOL ? A.mol[s] : 0;
   const int tbase = ti * M.nt;
+  Real pa0 = 0, pb0 = 0, pc0 = 0, pa1 = 0, pb1 = 0, pc1 = 0;
+  if (FAST) {
+    const int a = ti == 1 ? 2 : 0;
+    pa0 = s_fc[a][0];
+    pb0 = s_fc[a][1];
+    pc0 = s_fc[a][2];
+    pa1 = s_fc[a + 1][0];
+    pb1 = s_fc[a + 1][1];
+    pc1 = s_fc[a + 1][2];
+  }
   Real fx = 0, fy = 0, fz = 0;
-  for_stencil3(Rp, A.cstart, cell_coord(xi.x, Rp.cl[0], Rp.lx), cell_coord(xi.y, Rp.cl[1], Rp.ly),
-               cell_coord(xi.z, Rp.cl[2], Rp.lz), [&](int j) {
+  auto pair = [&](int j, int tj, double4 pj, double shx, double shy, double shz) {
     if (j == s || (MOL && A.mol[j] == mi)) return;
-    const int pq = tbase + A.type[j];
-    const int q0 = M.pfirst[pq], qn = M.pcount[pq];
-    if (qn == 0) return;
-    const double3 d = disp3(Rp, xi, A.pos4[j]);
-    const Real rx = (Real)d.x, ry = (Real)d.y, rz = (Real)d.z;
+    const Real rx = (Real)((pj.x + shx) - xi.x);
+    const Real ry = (Real)((pj.y + shy) - xi.y);
+    const Real rz = (Real)((pj.z + shz) - xi.z);
     const Real r2 = rx * rx + ry * ry + rz * rz;
     if (!(r2 > Real(0))) return;  // coincident particles: no direction
-    for (int q = q0; q < q0 + qn; ++q) {
-      const DTerm& tm = M.terms[q];
-      if (!(r2 <= TP<Real>::rc2(tm))) continue;
-      const Real c = pair_coef<Real>(tm, r2);
+    if (FAST) {
+      const bool m1 = tj == 1;
+      const Real pa = m1 ? pa1 : pa0, pb = m1 ? pb1 : pb0, pc = m1 ? pc1 : pc0;
+      const Real c = r2 <= pc ? lj_fast<Real, KM>(r2, pa, pb, pw) : Real(0);
       fx += c * rx;
       fy += c * ry;
       fz += c * rz;
+    } else {
+      const int pq = tbase + tj;
+      const int q0 = M.pfirst[pq], qn = M.pcount[pq];
+      for (int q = q0; q < q0 + qn; ++q) {
+        const DTerm& tm = M.terms[q];
+        if (!(r2 <= TP<Real>::rc2(tm))) continue;
+        const Real c = pair_coef<Real>(tm, r2);
+        fx += c * rx;
+        fy += c * ry;
+        fz += c * rz;
+      }
     }
-  });
+  };
+  constexpr int NB = FAST ? 4 : 2;  // candidates whose loads are in flight together
+  auto range = [&](int a, int b, double shx, double shy, double shz) {
+    for (int j0 = a; j0 < b; j0 += NB) {
+      int tj[NB];
+      double4 pj[NB];
+#pragma unroll
+      for (int v = 0; v < NB; ++v) {
+        const int j = min(j0 + v, b - 1);
+        tj[v] = A.type[j];
+        pj[v] = A.pos4[j];
+      }
+#pragma unroll
+      for (int v = 0; v < NB; ++v)
+        if (j0 + v < b) pair(j0 + v, tj[v], pj[v], shx, shy, shz);
+    }
+  };
+  const int cx = cell_coord(xi.x, Rp.cl[0], Rp.lx), cy = cell_coord(xi.y, Rp.cl[1], Rp.ly);
+  const int cz = cell_coord(xi.z, Rp.cl[2], Rp.lz);
+  const int lx = Rp.lx;
+#pragma unroll 1
+  for (int r = 0; r < 9; ++r) {
+    int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
+    double shz = 0.0, shy = 0.0;
+    if (z < 0 || z >= Rp.lz) {
+      if (!Rp.sz) continue;
+      shz = z < 0 ? -Rp.L[2] : Rp.L[2];
+      z = z < 0 ? z + Rp.lz : z - Rp.lz;
+    }
+    if (y < 0 || y >= Rp.ly) {
+      if (!Rp.sy) continue;
+      shy = y < 0 ? -Rp.L[1] : Rp.L[1];
+      y = y < 0 ? y + Rp.ly : y - Rp.ly;
+    }
+    const int rowc = Rp.cbase + (z * Rp.ly + y) * lx;
+    if (!Rp.sx) {
+      range(A.cstart[rowc + max(cx - 1, 0)], A.cstart[rowc + min(cx + 1, lx - 1) + 1], 0.0, shy,
+            shz);
+    } else if (cx == 0) {
+      range(A.cstart[rowc + lx - 1], A.cstart[rowc + lx], -Rp.L[0], shy, shz);
+      range(A.cstart[rowc], A.cstart[rowc + 2], 0.0, shy, shz);
+    } else if (cx == lx - 1) {
+      range(A.cstart[rowc + cx - 1], A.cstart[rowc + lx], 0.0, shy, shz);
+      range(A.cstart[rowc], A.cstart[rowc + 1], Rp.L[0], shy, shz);
+    } else {
+      range(A.cstart[rowc + cx - 1], A.cstart[rowc + cx + 2], 0.0, shy, shz);
+    }
+  }
   double F[3] = {(double)fx, (double)fy, (double)fz};
   const double X[3] = {xi.x, xi.y, xi.z};
   for (int q = 0; q < M.nwalls; ++q) {
@@ -3096,14 +3185,24 @@ void launch_bonds(rds_ctx* c, const Buf& cur) {
   hipLaunchKernelGGL(k_bonds, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream, b);
 }
 
-void launch_force3(rds_ctx* c, const StepArgs& a, int mode) {
+template <typename Real, bool MOL>
+void launch_force3_m(rds_ctx* c, const StepArgs& a, int mode) {
   const dim3 g((unsigned)c->n_sblocks), b(256);
+  if (!c->fast)
+    hipLaunchKernelGGL((k3_force<Real, MOL, 0>), g, b, 0, c->stream, a, mode, c->pw);
+  else if (c->pw == 3)
+    hipLaunchKernelGGL((k3_force<Real, MOL, 2>), g, b, 0, c->stream, a, mode, c->pw);
+  else
+    hipLaunchKernelGGL((k3_force<Real, MOL, 1>), g, b, 0, c->stream, a, mode, c->pw);
+}
+
+void launch_force3(rds_ctx* c, const StepArgs& a, int mode) {
   if (c->precision == RDS_FP64) {
-    if (c->has_mol) hipLaunchKernelGGL((k3_force<double, true>), g, b, 0, c->stream, a, mode);
-    else hipLaunchKernelGGL((k3_force<double, false>), g, b, 0, c->stream, a, mode);
+    if (c->has_mol) launch_force3_m<double, true>(c, a, mode);
+    else launch_force3_m<double, false>(c, a, mode);
   } else {
-    if (c->has_mol) hipLaunchKernelGGL((k3_force<float, true>), g, b, 0, c->stream, a, mode);
-    else hipLaunchKernelGGL((k3_force<float, false>), g, b, 0, c->stream, a, mode);
+    if (c->has_mol) launch_force3_m<float, true>(c, a, mode);
+    else launch_force3_m<float, false>(c, a, mode);
   }
 }
 
