@@ -17,10 +17,10 @@
 //              loops over its 3x3 stencil in LDS; the force is consumed at once
 //              by the Euler-Maruyama update, the boundary conditions and the
 //              binning of the new position (atomic per-cell histogram + rank)
-//   k_cellsum, k_cellscan
-//              cell starts from the histogram in chunks of <= 2048 cells (a
-//              replica spreads over many workgroups); histogram cleared for
-//              the next step, step counter advanced
+//   k_cellscan cell starts from the histogram in one pass, in chunks of <= 2048
+//              cells (a replica spreads over many workgroups) with a
+//              decoupled look-back between a replica's chunks; histogram
+//              cleared for the next step, step counter advanced
 //   k_scatter  counting-sort reorder of the new state into the other buffer;
 //              with reactions it also decides the particles whose rates need
 //              no neighbours and lists the others
@@ -132,7 +132,8 @@ struct DStatus {
   int err_lock;                // guards err_key / err_val together
   int err_lost;                // a failing lane could not take the lock
   unsigned long long events;
-  int integrity;               // bit 0: scatter index out of range, bit 1: histogram total
+  int integrity;               // bit 0: scatter index out of range, bit 1: histogram total,
+                               // bit 2: cell-start look-back gave up waiting
   int rebuild;                 // list mode: set when a particle moved more than skin/2
   unsigned long long rebuilds, overflow, entries;
   unsigned done;               // blocks of the list build that finished (reset by the last)
@@ -2026,12 +2027,21 @@ __global__ __launch_bounds__(256) void k_scatter_react(
 // Cell starts in chunks of at most CHUNK_CELLS cells of one replica
 // (chunk = (replica, first cell within the replica, cells, index of the
 // replica's first chunk)), so that large replicas spread over many
-// workgroups.  k_cellsum: particles per chunk.  k_cellscan: the chunk's base
-// (pbase + the sums of the replica's earlier chunks) plus a workgroup scan of
-// its histogram into cell starts; the histogram is cleared for the next
-// step and the step counter advanced (advance).
+// workgroups, in one pass: each chunk publishes its particle count in a
+// 64-bit status word as soon as it has summed its histogram, then adds the
+// counts of the replica's earlier chunks, one thread per chunk, for its base.
+// A chunk waits only on chunks of lower index, which are dispatched no later
+// than itself and publish without waiting on anything; the wait is still
+// bounded, and a chunk that gives up flags the status (the step then fails
+// loudly) instead of hanging.  The words are
+// double-buffered by buffer parity: this step's scan clears the other
+// parity's for the next.  The histogram is cleared for the next step and the
+// step counter advanced (advance).
 #define CHUNK_CELLS 2048
 #define CHUNK_THREADS 256
+#define LB_AGG (1ull << 62)             // word holds the chunk's count
+#define LB_VAL ((1ull << 62) - 1ull)
+#define LB_SPIN (1 << 22)
 
 __device__ __forceinline__ int block_sum_256(int v, int* s_red) {
   const int t = threadIdx.x;
@@ -2044,23 +2054,10 @@ __device__ __forceinline__ int block_sum_256(int v, int* s_red) {
   return tot;
 }
 
-__global__ __launch_bounds__(CHUNK_THREADS) void k_cellsum(const int4* __restrict__ chunks,
-                                                           const DRep* __restrict__ rep,
-                                                           const int* __restrict__ count,
-                                                           int* __restrict__ csum) {
-  __shared__ int s_red[4];
-  const int4 ch = chunks[blockIdx.x];
-  const int c0 = rep[ch.x].cbase + ch.y;
-  int v = 0;
-  for (int i = threadIdx.x; i < ch.z; i += CHUNK_THREADS) v += count[c0 + i];
-  const int tot = block_sum_256(v, s_red);
-  if (threadIdx.x == 0) csum[blockIdx.x] = tot;
-}
-
 __global__ __launch_bounds__(CHUNK_THREADS) void k_cellscan(
     const int4* __restrict__ chunks, const DRep* __restrict__ rep, int* __restrict__ count,
-    int* __restrict__ cstart, const int* __restrict__ csum, long long* step, int advance,
-    DStatus* status) {
+    int* __restrict__ cstart, unsigned long long* lb, unsigned long long* lb_next,
+    long long* step, int advance, DStatus* status) {
   __shared__ int s_red[4];
   __shared__ int s_wsum[CHUNK_THREADS / 64];
   __shared__ __align__(16) int s_v[CHUNK_CELLS];
@@ -2069,16 +2066,13 @@ __global__ __launch_bounds__(CHUNK_THREADS) void k_cellscan(
   const DRep& R = rep[ch.x];
   const int c0 = R.cbase + ch.y;
   // the chunk's histogram through LDS, so that global loads and stores are
-  // coalesced while each thread scans PER consecutive cells; the particles
-  // in the replica's chunks before this one are loaded in the same round
+  // coalesced while each thread scans PER consecutive cells
   constexpr int PER = CHUNK_CELLS / CHUNK_THREADS;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int c = i * CHUNK_THREADS + t;
     s_v[c] = c < ch.z ? count[c0 + c] : 0;
   }
-  int pre = 0;
-  for (int k = ch.w + t; k < b; k += CHUNK_THREADS) pre += csum[k];
   __syncthreads();
   int v[PER];
 #pragma unroll
@@ -2092,6 +2086,27 @@ __global__ __launch_bounds__(CHUNK_THREADS) void k_cellscan(
   int tsum = 0;
 #pragma unroll
   for (int i = 0; i < PER; ++i) tsum += v[i];
+  // publish the chunk's count at once, then sum the replica's earlier
+  // chunks' counts, one thread per chunk, each waiting (boundedly) until that
+  // chunk has published
+  const int tot = block_sum_256(tsum, s_red);
+  if (t == 0)
+    __hip_atomic_store(lb + b, LB_AGG | (unsigned long long)tot, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  int pre = 0;
+  for (int k = ch.w + t; k < b; k += CHUNK_THREADS) {
+    unsigned long long x = 0ull;
+    for (int spins = 0; !((x = __hip_atomic_load(lb + k, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT)) & LB_AGG);) {
+      if (++spins > LB_SPIN) {
+        atomicOr(&status->integrity, 4);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    pre += (int)(x & LB_VAL);
+  }
+  if (t == 0) lb_next[b] = 0ull;
   const int base = R.pbase + block_sum_256(pre, s_red);
   int inc = tsum;
 #pragma unroll
@@ -2888,7 +2903,7 @@ struct rds_ctx {
   int* count = nullptr;
   int* cstart = nullptr;
   int4* d_chunks = nullptr;  // cell-start chunks (replica, first cell, cells, first chunk)
-  int* d_csum = nullptr;     // particles per chunk
+  unsigned long long* d_lb = nullptr;  // [2][n_chunks] look-back words of the scan
   int64_t n_chunks = 0;
   int* wl = nullptr;         // reaction worklist (cell mode), 256 entries per slot block
   int* wl_n = nullptr;
@@ -3061,14 +3076,14 @@ inline unsigned nblk_gs(int64_t n) { return std::min(nblk(n, 256), 4096u); }
 // kernel ids for timing
 enum { KT_FORCE = 0, KT_SCAN = 1, KT_SCATTER = 2, KT_REACT = 3, KT_BONDS = 4, KT_LIST = 5, KT_N = 6 };
 
-// Cell starts from the histogram (chunked scan); advances the step counter
-// when `advance`.
-void launch_cell_starts(rds_ctx* c, int advance) {
-  hipLaunchKernelGGL(k_cellsum, dim3((unsigned)c->n_chunks), dim3(CHUNK_THREADS), 0, c->stream,
-                     c->d_chunks, c->d_rep, c->count, c->d_csum);
-  hipLaunchKernelGGL(k_cellscan, dim3((unsigned)c->n_chunks), dim3(CHUNK_THREADS), 0, c->stream,
-                     c->d_chunks, c->d_rep, c->count, c->cstart, c->d_csum, c->d_step, advance,
-                     c->d_status);
+// Cell starts from the histogram (one-pass chunked scan); advances the step
+// counter when `advance`.  p: the buffer parity whose look-back words the
+// scan uses (it clears the other parity's).
+void launch_cell_starts(rds_ctx* c, int advance, int p) {
+  const size_t K = (size_t)c->n_chunks;
+  hipLaunchKernelGGL(k_cellscan, dim3((unsigned)K), dim3(CHUNK_THREADS), 0, c->stream,
+                     c->d_chunks, c->d_rep, c->count, c->cstart, c->d_lb + (size_t)p * K,
+                     c->d_lb + (size_t)(p ^ 1) * K, c->d_step, advance, c->d_status);
 }
 
 template <bool MOL>
@@ -3234,7 +3249,7 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
     launch_force3(c, a, MODE_STEP);
     mark(KT_FORCE, false);
     mark(KT_SCAN, true);
-    launch_cell_starts(c, 1);
+    launch_cell_starts(c, 1, c->parity);
     mark(KT_SCAN, false);
     mark(KT_SCATTER, true);
     hipLaunchKernelGGL(k3_scatter, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
@@ -3284,7 +3299,7 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
   launch_force(c, a, MODE_STEP);
   mark(KT_FORCE, false);
   mark(KT_SCAN, true);
-  launch_cell_starts(c, 1);
+  launch_cell_starts(c, 1, c->parity);
   mark(KT_SCAN, false);
   // reactions: the scatter decides the particles whose rates need no
   // neighbours and lists the others for k_react_walk, which reads the
@@ -3564,7 +3579,7 @@ int rds_ctx_destroy(rds_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   std::vector<void*> ptrs = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->d_sblocks,
                              c->upos, c->ucell, c->urank, c->count, c->cstart, c->d_step,
-                             c->d_chunks, c->d_csum, c->wl, c->wl_n,
+                             c->d_chunks, c->d_lb, c->wl, c->wl_n,
                              c->d_status, c->boff, c->bpart, c->br0, c->bu0, c->fbond,
                              c->slot_of, c->clabel, c->aoff, c->aent, c->apart, c->aprm, c->nl4, c->ncnt, c->xref, c->scell, c->spos,
                              c->stype, c->spid, c->smol};
@@ -3847,7 +3862,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     }
     c->n_chunks = (int64_t)ch.size();
     A_(c->d_chunks, ch.size());
-    A_(c->d_csum, ch.size());
+    A_(c->d_lb, 2 * ch.size());
     HIP_TRY(hipMemcpy(c->d_chunks, ch.data(), sizeof(int4) * ch.size(), hipMemcpyHostToDevice));
   }
   A_(c->d_step, 1);
@@ -3876,6 +3891,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     HIP_TRY(hipMemcpy(c->d_sblocks, sb.data(), sizeof(int4) * sb.size(), hipMemcpyHostToDevice));
   }
   HIP_TRY(hipMemset(c->count, 0, sizeof(int) * C));
+  HIP_TRY(hipMemset(c->d_lb, 0, sizeof(unsigned long long) * 2 * (size_t)c->n_chunks));
   {
     int nt = (int)N;
     HIP_TRY(hipMemcpy(c->cstart + C, &nt, sizeof(int), hipMemcpyHostToDevice));
@@ -3984,13 +4000,18 @@ int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32
   memset(&st, 0, sizeof st);
   st.err_key = ~0ull;
   HIP_TRY(hipMemcpyAsync(c->d_status, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+  // the load's scan uses parity 0 and clears parity 1's look-back words for
+  // the first step; parity 0's may hold a previous run's last words
+  if (!c->list)
+    HIP_TRY(hipMemsetAsync(c->d_lb, 0, sizeof(unsigned long long) * 2 * (size_t)c->n_chunks,
+                           c->stream));
   if (c->ndim == 3) {
     hipLaunchKernelGGL(k3_load, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, x, types,
                        c->has_mol ? mol : nullptr, a.pos4, a.type, a.pid,
                        c->has_mol ? a.mol : nullptr);
     hipLaunchKernelGGL(k3_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep,
                        c->d_rep_of, a.pos4, c->ucell, c->urank, c->count);
-    launch_cell_starts(c, 0);
+    launch_cell_starts(c, 0, 0);
     hipLaunchKernelGGL(k3_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos4,
                        c->ucell, c->urank, c->cstart, a.type, a.pid, a.mol, b.pos4, b.type,
                        b.pid, b.mol, c->slot_of, (int)c->n_cells, c->d_status);
@@ -4001,7 +4022,7 @@ int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32
   } else {
     hipLaunchKernelGGL(k_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep,
                        c->d_rep_of, a.pos, c->ucell, c->urank, c->count, nullptr, c->cpos);
-    launch_cell_starts(c, 0);
+    launch_cell_starts(c, 0, 0);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos,
                        c->ucell, c->urank, c->cstart, a.type, a.pid, a.mol, b.pos, b.type, b.pid,
                        b.mol, c->slot_of, (int)c->n_cells, c->d_status);
