@@ -288,11 +288,13 @@ def c1_leg(runs=3, steps=2000):
                 unit='particle-steps/s')
 
 
-def three_d_leg(n=262144, steps=200, warmup=50):
+def three_d_leg(n=262144, steps=200, warmup=50, cpu_n=2048, cpu_seconds=8.0, cpu=True):
     """The three-dimensional engine (n_dim = 3, 27-cell stencil): the
     two-species LJ mixture of the ensemble in a periodic cube at density 0.6,
     N = 262,144, graphs of 50 steps, in mixed precision and in FP64, in
-    particle-steps/s."""
+    particle-steps/s; beside it the oracle in reference-faithful mode (Verlet
+    list, rs = 2 rc, ratio 2, refresh every 10 steps, mt19937 stream, FP64)
+    on one core, on the same model at N = cpu_n for about cpu_seconds."""
     import torch
     from pyrdsim_amd.engine import Engine, build_model, make_replica
     s = workloads.lj3(n=n)
@@ -307,6 +309,23 @@ def three_d_leg(n=262144, steps=200, warmup=50):
         out[prec] = n * steps / time_engine(e, steps, warmup)
         e.close()
     torch.cuda.synchronize()
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        from oracle import Oracle
+        sc = workloads.lj3(n=cpu_n)
+        o = Oracle(sc['x'], sc['types'], sc['L'], sc['periodic'], sc['dt'], m['tau_by_tauD'],
+                   m['interactions'], m['reactions'], seed=2001, rng='mt19937', nl='faithful',
+                   ratio=2, nl_freq=10)
+        o.init()
+        done = 0
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            o.run(done + 1, 10)
+            done += 10
+        el = time.perf_counter() - t0
+        out['cpu_oracle_faithful_1core'] = cpu_n * done / el
+        out['cpu_sample'] = 'oracle, reference-faithful FP64, N=%d x %d steps (%.1f s)' % (
+            cpu_n, done, el)
     return out
 
 
@@ -574,7 +593,7 @@ def main():
         print('# c1: %s' % c1, file=sys.stderr, flush=True)
     three_d = None
     if args.three_d:
-        three_d = three_d_leg()
+        three_d = three_d_leg(cpu=not args.no_cpu)
         print('# 3-D: %s' % three_d, file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
