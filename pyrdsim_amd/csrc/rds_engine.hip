@@ -1498,104 +1498,31 @@ __device__ __forceinline__ int react_decide(const StepArgs& A, const DRep& Rp, c
   return M.rx[chosen].product;
 }
 
-// New type of slot s of replica Rp (pre-reaction types in A.type): its
-// neighbour counts within each reaction's r0 when its type is the reactant of
-// a catalysed or inhibited reaction, then the decision.
-template <bool MOL>
-__device__ int react_slot(const StepArgs& A, const DRep& Rp, int s) {
-  const DModel& M = A.models[Rp.model];
-  const int nrx = M.nrx;
-  const int ti = A.type[s];
-  if (nrx == 0) return ti;
-  const int p = A.pid[s];
-  const long long step = *A.step;  // already advanced: the reference's l+1
-  bool need = false;
-  unsigned nbr_types = 0u;  // neighbour types that any of ti's reactions counts
-  for (int q = 0; q < nrx; ++q) {
-    const DRx& rx = M.rx[q];
-    if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
-    need = true;
-    const int a = rx.kind == RDS_RX_CATALYZED ? rx.catalyst : rx.inh;
-    const int b = rx.kind == RDS_RX_CATALYZED ? -1 : rx.inh2;
-    if (a >= 0 && a < 32) nbr_types |= 1u << a;
-    if (b >= 0 && b < 32) nbr_types |= 1u << b;
-  }
-  unsigned long long c_lo = 0ull, c_hi = 0ull;
-  if (need) {
-    const double2 xi = rd_pos(A.pos, s, A.cpos);
-    const int mi = MOL ? A.mol[s] : 0;
-    // the cell the slot is sorted into (list mode: of the last rebuild; cells
-    // are then at least range + skin wide, so the 3x3 stencil still holds
-    // every particle within the range)
-    int cx, cy;
-    if (A.scell) {
-      const int c = A.scell[s] - Rp.cbase;
-      cy = c / Rp.lx;
-      cx = c - cy * Rp.lx;
-    } else {
-      cx = cell_coord(xi.x, Rp.cl[0], Rp.lx);
-      cy = cell_coord(xi.y, Rp.cl[1], Rp.ly);
-    }
-    for (int oy = -1; oy <= 1; ++oy) {
-      int y = cy + oy;
-      if (y < 0 || y >= Rp.ly) {
-        if (!Rp.sy) continue;
-        y = (y < 0) ? y + Rp.ly : y - Rp.ly;
-      }
-      for (int ox = -1; ox <= 1; ++ox) {
-        int x = cx + ox;
-        if (x < 0 || x >= Rp.lx) {
-          if (!Rp.sx) continue;
-          x = (x < 0) ? x + Rp.lx : x - Rp.lx;
-        }
-        const int c = Rp.cbase + y * Rp.lx + x;
-        const int j1 = A.cstart[c + 1];
-        for (int j = A.cstart[c]; j < j1; ++j) {
-          if (j == s) continue;
-          // only neighbours of a counted type need a distance
-          const int tj = A.type[j];
-          if (tj < 0 || tj >= 32 || !((nbr_types >> tj) & 1u)) continue;
-          // the reference's neighbour list holds other molecules' particles
-          // and the bonded partners (pyrdsim.pyx:1513, 1523)
-          if (MOL && A.mol[j] == mi) {
-            bool bonded = false;
-            if (A.boff) {
-              const int pj = A.pid[j];
-              for (int b = A.boff[p]; b < A.boff[p + 1]; ++b) bonded |= A.bpart[b] == pj;
-            }
-            if (!bonded) continue;
-          }
-          const double2 xj = rd_pos(A.pos, j, A.cpos);
-          double dx = xj.x - xi.x, dy = xj.y - xi.y;
-          if (Rp.px) dx = min_image(dx, Rp.L[0]);
-          if (Rp.py) dy = min_image(dy, Rp.L[1]);
-          const double d2 = dx * dx + dy * dy;
-#pragma unroll 1
-          for (int q = 0; q < nrx; ++q) {
-            const DRx& rx = M.rx[q];
-            if (rx.kind == RDS_RX_LINEAR || rx.reactant != ti) continue;
-            const bool match = (rx.kind == RDS_RX_CATALYZED) ? (tj == rx.catalyst)
-                                                             : (tj == rx.inh || tj == rx.inh2);
-            // d < r0 with d = sqrt(d2) as the reference computes it; away
-            // from r0^2 (relative 1e-14) d2 alone decides it
-            const bool within = d2 < rx.r0sq_lo ||
-                                (d2 <= rx.r0sq_hi && sqrt(d2) < rx.r0);
-            if (match && within) cnt_add(c_lo, c_hi, q);
-          }
-        }
-      }
-    }
-  }
-  return react_decide(A, Rp, M, ti, p, step, c_lo, c_hi);
-}
-
-// List mode: every slot, one thread each, in per-replica slot blocks.
-template <bool MOL>
-__global__ __launch_bounds__(256) void k_react(const StepArgs A) {
+// List mode, first half of the reaction step, per slot block: every particle
+// whose rates need no neighbours is decided here (pyrdsim.pyx:2103-2222); the
+// others are listed for k_react_walk, as k_scatter_react lists them in cell
+// mode.
+__global__ __launch_bounds__(256) void k_react_list(const StepArgs A) {
+  __shared__ int s_wcnt[4];
   const int4 B = A.sblocks[blockIdx.x];  // (replica, first slot, end slot)
+  const DRep& Rp = A.rep[B.x];
+  const DModel& M = A.models[Rp.model];
+  const long long step = *A.step;  // already advanced: the reference's l+1
   const int s = B.y + (int)threadIdx.x;
-  if (s >= B.z) return;
-  A.type_out[s] = react_slot<MOL>(A, A.rep[B.x], s);
+  bool walker = false;
+  if (s < B.z) {
+    const int t = A.type[s];
+    if ((M.walk_types >> t) & 1u) walker = true;
+    else A.type_out[s] = M.nrx > 0 ? react_decide(A, Rp, M, t, A.pid[s], step, 0ull, 0ull) : t;
+  }
+  const unsigned long long m = __ballot(walker);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) s_wcnt[wv] = __popcll(m);
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < wv; ++w) base += s_wcnt[w];
+  if (walker) A.wl[blockIdx.x * 256 + base + __popcll(m & ((1ull << lane) - 1ull))] = s;
+  if (threadIdx.x == 0) A.wl_n[blockIdx.x] = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
 }
 
 // Cell mode: the slots k_scatter_react listed (reactants of a catalysed or
@@ -1650,8 +1577,18 @@ __global__ __launch_bounds__(256, 6) void k_react_walk(const StepArgs A, int n_b
     if (qa >= 0) { lo_a = M.rx[qa].r0sq_lo; hi_a = M.rx[qa].r0sq_hi; r0_a = M.rx[qa].r0; }
     if (qb >= 0) { lo_b = M.rx[qb].r0sq_lo; hi_b = M.rx[qb].r0sq_hi; r0_b = M.rx[qb].r0; }
     unsigned long long c_lo = 0ull, c_hi = 0ull;  // packed counts (cnt_of)
-    const int cx = cell_coord(xi.x, cl0, lx);
-    const int cy = cell_coord(xi.y, cl1, ly);
+    // the slot's cell: in list mode the cell of the last rebuild (cells are
+    // then at least range + skin wide, so its stencil holds every particle
+    // within r0; distances take the minimum image)
+    int cx, cy;
+    if (A.scell) {
+      const int c = A.scell[s] - cbase;
+      cy = c / lx;
+      cx = c - cy * lx;
+    } else {
+      cx = cell_coord(xi.x, cl0, lx);
+      cy = cell_coord(xi.y, cl1, ly);
+    }
     // the stencil rows' cells cx-1 .. cx+1, each as at most two contiguous
     // slot ranges [ra0, ra1) and [rb0, rb1); a row outside a reflecting wall
     // is empty.  The cell-start loads of all three rows are issued together.
@@ -3286,10 +3223,13 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
       r.type = cur.type;
       r.type_out = nxt.type;
       mark(KT_REACT, true);
+      const int nb = (int)c->n_sblocks;
+      hipLaunchKernelGGL(k_react_list, dim3((unsigned)nb), dim3(256), 0, c->stream, r);
+      const unsigned g = (unsigned)((nb + 3) / 4);  // one wave per slot block
       if (c->has_mol)
-        hipLaunchKernelGGL((k_react<true>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
+        hipLaunchKernelGGL((k_react_walk<true>), dim3(g), dim3(256), 0, c->stream, r, nb);
       else
-        hipLaunchKernelGGL((k_react<false>), dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, r);
+        hipLaunchKernelGGL((k_react_walk<false>), dim3(g), dim3(256), 0, c->stream, r, nb);
       mark(KT_REACT, false);
     }
     c->parity ^= 1;
