@@ -1065,18 +1065,20 @@ void k_force(const StepArgs A, int mode, int pw) {
       }
       if (LIST) {
         if (resident) {
-          // four partners per chunk, the next chunk loaded while this one is
-          // computed; padding entries repeat a staged index, masked
+          // four partners per chunk, the chunk after next loaded while this
+          // one is computed; padding entries repeat a staged index, masked
           const int nch = (nlist + 3) >> 2;
           uint2 ck = nch > 0 ? lp[0] : make_uint2(0u, 0u);
+          uint2 ck1 = nch > 1 ? lp[NS] : ck;
           for (int k = 0; k < nch; ++k) {
-            const uint2 nx = lp[(size_t)min(k + 1, nch - 1) * NS];
+            const uint2 nx = lp[(size_t)min(k + 2, nch - 1) * NS];
             const int b4 = 4 * k;
             pair((int)(ck.x & 0xffffu), true, true);
             pair((int)(ck.x >> 16), b4 + 1 < nlist, true);
             pair((int)(ck.y & 0xffffu), b4 + 2 < nlist, true);
             pair((int)(ck.y >> 16), b4 + 3 < nlist, true);
-            ck = nx;
+            ck = ck1;
+            ck1 = nx;
           }
         } else {
           // lists are ascending in staged index: take this round's entries
