@@ -64,19 +64,22 @@ def test_mixed_precision_rdf_within_one_percent(gpu, name):
         assert np.abs(res[prec][2] - res['fp64'][2]).sum() <= 0.05, prec
 
 
-@pytest.mark.parametrize('name', ['kalj', 'wall'])
+@pytest.mark.parametrize('name', ['kalj', 'wall', 'patchy'])
 def test_full_size_configurations(gpu, name):
-    """The two single-system configurations at their full sizes (LJ mixture
-    N = 262,144 at density 0.6; wall + reactions N = 65,536 at density 0.4):
-    with the same Philox streams, FP32-force runs (FP64 or compensated
-    positions) stay within 2e-5 sigma of the FP64 run after 100 steps (the
-    seeded states hold close pairs whose forces reach 1e3), keep every
-    particle in the box, and make the same reaction events up to flips at
-    pair distances within the runs' difference of r0."""
+    """The three single-system configurations at their full sizes (LJ
+    mixture N = 262,144 at density 0.6; wall + reactions N = 65,536 at density
+    0.4; patchy colloids, 16,384 molecules of four bonded beads at packing
+    fraction 0.3): with the same Philox streams, FP32-force runs (FP64 or
+    compensated positions) stay within 2e-5 sigma of the FP64 run after 100
+    steps (the seeded states hold close pairs whose forces reach 1e3), keep
+    every particle in the box, and make the same reaction events up to flips
+    at pair distances within the runs' difference of r0."""
     from pyrdsim_amd import workloads
-    s = workloads.kalj() if name == 'kalj' else workloads.wall()
+    s = {'kalj': workloads.kalj, 'wall': workloads.wall, 'patchy': workloads.patchy}[name]()
     if name == 'wall':
         s['types'][::5] = 1
+    if name == 'patchy':
+        s['types'][2::4] = 2   # some active patches: tanh attraction and catalysis
     out, ev = {}, {}
     for prec in ('fp64', 'mixed', 'compensated'):
         eng = engine_for(s, precision=prec, seed=9, graph_steps=25)
