@@ -646,6 +646,15 @@ __device__ void tile_stage(const TileSh<R>& sh, int NR, int p0, int cap,
   }
 }
 
+// The slot after the last staged particle of a round: position (0, 0), type
+// 0 (any finite position and valid type; what is read there is masked).
+__device__ __forceinline__ void stage_sentinel(double2* s_pos, int* s_ty, int* s_mol, int k,
+                                               bool mol) {
+  s_pos[k] = make_double2(0.0, 0.0);
+  s_ty[k] = 0;
+  if (mol) s_mol[k] = -1;
+}
+
 // Home particle of lane `lane` of wave wv in batch b, and its three windows.
 // The tile's home particles, taken cell by cell in row-major order of the
 // home cells, are dealt to the waves 64 at a time: home number
@@ -848,6 +857,14 @@ __device__ __forceinline__ Real lj_fast(Real r2, Real pa, Real pb, int pw) {
   return (pa * inv) * fma(s, s, -s);
 }
 
+// n = 6 with the constants folded: c_lj / r^2 * s (s - 1), s = r0^6 / r^6,
+// equals r^-8 * (A r^-6 - B) with A = c_lj r0^12 and B = c_lj r0^6.
+__device__ __forceinline__ float lj_fast6(float r2, float A, float B) {
+  const float inv = rcp_(r2);
+  const float i2 = inv * inv;
+  return (i2 * i2) * fmaf(A, i2 * inv, -B);
+}
+
 // 0 < r2 <= rc2 for r2 >= +0 or NaN and rc2 >= 0, as one unsigned compare: the
 // bit patterns of non-negative floats are ordered like their values, +0 maps
 // to 0 (and wraps to the largest pattern), NaN lies above every finite value.
@@ -883,7 +900,9 @@ void k_force(const StepArgs A, int mode, int pw) {
   constexpr bool FAST = KM != 0;
   extern __shared__ __align__(16) char smem[];
   const int CAPT = A.cap;
-  const Staging G = staging(smem, CAPT);
+  // one slot beyond the round capacity: a finite sentinel after the staged
+  // particles, read (masked) by the second candidate of a window's last trip
+  const Staging G = staging(smem, CAPT + 1);
   double2* s_pos = G.pos;
   int* s_ty = G.ty;
   int* s_mol = G.mol;
@@ -917,22 +936,34 @@ void k_force(const StepArgs A, int mode, int pw) {
   __shared__ double s_fp[FAST ? 2 : 1][2];
   // FP32 pair arithmetic: the same constants as float4 entries (a, b) at
   // byte 32 a + 16 b, the staged types holding 16 b, so that a candidate's
-  // constants are one LDS read at its lane's row plus its staged type
+  // constants are one LDS read at its lane's row plus its staged type; for
+  // n = 6 the entries hold (c_lj r0^12, c_lj r0^6, cutoff^2), rounded once
+  // from FP64
   constexpr int TSH = (FAST && sizeof(Real) == 4) ? 4 : 0;
+  constexpr bool LJ6 = TSH && KM == 2;
+  // FP32 pair arithmetic on FP64 staging, stencil search: windows walked one
+  // at a time
+  constexpr bool WIN = TSH && !LIST && PM == 0;
   __shared__ __align__(16) float4 s_fc4[TSH ? 4 : 1];
   if (FAST && t < 4) {
     const int a = t >> 1, bb = t & 1, q = a * nt + bb;
     Real v0 = 0, v1 = 0, v2 = 0;
+    double A6 = 0.0, B6 = 0.0;
     if (a < nt && bb < nt && M.pcount[q] > 0) {
       const DTerm& tm = M.terms[M.pfirst[q]];
       v0 = TP<Real>::c_lj(tm);
       v1 = TP<Real>::r0sq(tm);
       v2 = TP<Real>::rc2(tm);
+      const double r6 = tm.r0sq * tm.r0sq * tm.r0sq;
+      B6 = tm.c_lj * r6;
+      A6 = B6 * r6;
     }
     s_fc[t][0] = v0;
     s_fc[t][1] = v1;
     s_fc[t][2] = v2;
-    if (TSH) s_fc4[t] = make_float4((float)v0, (float)v1, (float)v2, 0.f);
+    if (TSH)
+      s_fc4[t] = LJ6 ? make_float4((float)A6, (float)B6, (float)v2, 0.f)
+                     : make_float4((float)v0, (float)v1, (float)v2, 0.f);
     if (bb == 0) {
       s_fp[a][0] = Rp.fpre[a];
       s_fp[a][1] = Rp.npre[a];
@@ -945,6 +976,7 @@ void k_force(const StepArgs A, int mode, int pw) {
   if (resident) {
     tile_stage<R, MOL, PM, TSH>(sh, NR, 0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp, scell,
                            PM);
+    if (t == 0) stage_sentinel(s_pos, s_ty, s_mol, total, MOL);
     __syncthreads();
   }
   long long step_next = 0;
@@ -1022,7 +1054,9 @@ void k_force(const StepArgs A, int mode, int pw) {
           pb = m1 ? pb1 : pb0;
           pc = m1 ? pc1 : pc0;
         }
-        Real c = lj_fast<Real, KM>(r2, pa, pb, pw);
+        Real c;
+        if constexpr (LJ6) c = lj_fast6(r2, pa, pb);
+        else c = lj_fast<Real, KM>(r2, pa, pb, pw);
         // atomic models: r2 > 0 excludes the particle itself, inside in_range
         const bool take = MOL ? (other && r2 <= pc) : in_range(r2, pc);
         c = (live && take) ? c : Real(0);
@@ -1061,6 +1095,7 @@ void k_force(const StepArgs A, int mode, int pw) {
         __syncthreads();
         tile_stage<R, MOL, PM, TSH>(sh, NR, p0, CAPT, A.pos, A.type, A.mol, s_pos, s_ty, s_mol, Rp,
                                scell, PM);
+        if (t == 0) stage_sentinel(s_pos, s_ty, s_mol, min(total - p0, CAPT), MOL);
         __syncthreads();
       }
       if (LIST) {
@@ -1093,7 +1128,30 @@ void k_force(const StepArgs A, int mode, int pw) {
           }
         }
       }
-      {
+      if constexpr (WIN) {
+        // the three row windows (clipped to this LDS round) one after the
+        // other, two consecutive staged particles per trip: the addresses
+        // advance linearly, at the cost of a wave running each window's
+        // longest lane.  Past a window's end the second candidate is the next
+        // staged particle or the sentinel, finite either way, and masked.
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int lo = max(h.lo[d], p0) - p0;
+          const int hi = min(h.hi[d], p0 + CAPT) - p0;
+          const double2* pq = s_pos + lo;
+          const double2* const pe = s_pos + hi;
+          const double2* const pl = pe - 1;  // a pair with its second past the end
+          const int* tq = s_ty + lo;
+          const int* mq = s_mol + lo;
+          for (; pq < pe; pq += 2, tq += 2, mq += MOL ? 2 : 0) {
+            const double2 q0 = pq[0], q1 = pq[1];
+            const int t0 = tq[0], t1 = tq[1];
+            const int m0 = MOL ? mq[0] : 0, m1 = MOL ? mq[1] : 0;
+            pairv((Real)(q0.x - xs.x), (Real)(q0.y - xs.y), t0, m0, true, false);
+            pairv((Real)(q1.x - xs.x), (Real)(q1.y - xs.y), t1, m1, pq < pl, false);
+          }
+        }
+      } else {
         // the three row windows (clipped to this LDS round) as one sequence,
         // so that a lane's trip count is its total; two candidates per trip,
         // past the end the second one repeats a staged index with its
@@ -3470,7 +3528,8 @@ bool build_tiles(rds_ctx* c, int rows, int target) {
   cap = std::max(cap, 256);
   cap = std::min(cap, (65536 - 256) / per / 64 * 64);
   c->cap = cap;
-  c->smem_bytes = (size_t)cap * per;
+  // the force kernel's staging holds one sentinel slot beyond the capacity
+  c->smem_bytes = (size_t)(cap + 1) * per;
   return true;
 }
 
