@@ -1296,8 +1296,8 @@ void k_force(const StepArgs A, int mode, int pw) {
     } else {
       const int cx = cell_coord_fast(v[0], Rp.cl[0], Rp.icl[0], Rp.lx);
       const int cyy = cell_coord_fast(v[1], Rp.cl[1], Rp.icl[1], Rp.ly);
+      // (the scatter recomputes the cell from the stored position)
       const int c = Rp.cbase + cyy * Rp.lx + cx;
-      A.ucell[s] = c;
       // ranks in the new cells, one atomic per run of lanes with the same new
       // cell: the lanes of a batch are a prefix of the wave (the valid home
       // particles), dealt cell by cell, and most particles stay in their cell
@@ -1939,26 +1939,40 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan(const DRep* __restrict__ re
 
 // Counting-sort reorder of the new state into the other buffer, one thread
 // per slot.
+// The new cell of a slot, from its stored new position: the same FP64 value
+// and the same routine as the force kernel's binning, hence the same cell.
+__device__ __forceinline__ int new_cell(const DRep& Rp, double2 raw, int cpos) {
+  double2 v = raw;
+  if (cpos) {
+    float4 f;
+    __builtin_memcpy(&f, &raw, sizeof(f));
+    v = make_double2((double)f.x + (double)f.z, (double)f.y + (double)f.w);
+  }
+  const int cx = cell_coord_fast(v.x, Rp.cl[0], Rp.icl[0], Rp.lx);
+  const int cy = cell_coord_fast(v.y, Rp.cl[1], Rp.icl[1], Rp.ly);
+  return Rp.cbase + cy * Rp.lx + cx;
+}
+
+// Counting-sort reorder into the other buffer, over the per-replica slot
+// blocks (sblocks: (replica, first slot, end slot)).
 __global__ __launch_bounds__(256) void k_scatter(
-    int n_total, const double2* __restrict__ upos, const int* __restrict__ ucell,
-    const int* __restrict__ urank, const int* __restrict__ cstart, const int* __restrict__ type,
+    const int4* __restrict__ sblocks, const DRep* __restrict__ rep, int cpos, int n_total,
+    const double2* __restrict__ upos, const int* __restrict__ urank,
+    const int* __restrict__ cstart, const int* __restrict__ type,
     const int* __restrict__ pid, const int* __restrict__ mol, double2* __restrict__ npos,
     int* __restrict__ ntype, int* __restrict__ npid, int* __restrict__ nmol,
-    int* __restrict__ slot_of, int n_cells, DStatus* status) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_total) return;
-  const int c = ucell[s];
-  if (c < 0 || c >= n_cells) {
-    atomicOr(&status->integrity, 1);
-    return;
-  }
-  const int d = cstart[c] + urank[s];
+    int* __restrict__ slot_of, DStatus* status) {
+  const int4 B = sblocks[blockIdx.x];
+  const int s = B.y + (int)threadIdx.x;
+  if (s >= B.z) return;
+  const double2 u = upos[s];
+  const int d = cstart[new_cell(rep[B.x], u, cpos)] + urank[s];
   if (d < 0 || d >= n_total) {
     atomicOr(&status->integrity, 1);
     return;
   }
   const int p = pid[s];
-  npos[d] = upos[s];
+  npos[d] = u;
   ntype[d] = type[s];
   npid[d] = p;
   if (nmol) nmol[d] = mol[s];
@@ -1973,11 +1987,11 @@ __global__ __launch_bounds__(256) void k_scatter(
 // the others are listed for k_react_walk in the block's own segment of the
 // worklist, compacted within the block.
 __global__ __launch_bounds__(256) void k_scatter_react(
-    const double2* __restrict__ upos, const int* __restrict__ ucell,
-    const int* __restrict__ urank, const int* __restrict__ cstart, const int* __restrict__ type,
+    const double2* __restrict__ upos, const int* __restrict__ urank,
+    const int* __restrict__ cstart, const int* __restrict__ type,
     const int* __restrict__ pid, const int* __restrict__ mol, double2* __restrict__ npos,
     int* __restrict__ ntype, int* __restrict__ npid, int* __restrict__ nmol,
-    int* __restrict__ slot_of, int n_cells, int* __restrict__ ntype0, const StepArgs A) {
+    int* __restrict__ slot_of, int* __restrict__ ntype0, const StepArgs A) {
   __shared__ int s_wcnt[4];
   const int4 B = A.sblocks[blockIdx.x];  // (replica, first slot, end slot)
   const DRep& Rp = A.rep[B.x];
@@ -1989,24 +2003,21 @@ __global__ __launch_bounds__(256) void k_scatter_react(
   bool walker = false;
   int d = -1;
   if (s < B.z) {
-    const int c = ucell[s];
-    if (c < 0 || c >= n_cells) {
+    const double2 u = upos[s];
+    d = cstart[new_cell(Rp, u, A.cpos)] + urank[s];
+    if (d < 0 || d >= A.n_total) {
       atomicOr(&A.status->integrity, 1);
-    } else {
-      d = cstart[c] + urank[s];
-      if (d < 0 || d >= A.n_total) {
-        atomicOr(&A.status->integrity, 1);
-        d = -1;
-      }
+      d = -1;
     }
     if (d >= 0) {
       const int p = pid[s];
       const int t = type[s];
-      npos[d] = upos[s];
+      npos[d] = u;
       npid[d] = p;
       if (nmol) nmol[d] = mol[s];
       if (slot_of) slot_of[p] = d;
-      ntype0[d] = t;
+      // the pre-reaction types, read by the walk only: models that have one
+      if (walk_types) ntype0[d] = t;
       if ((walk_types >> t) & 1u) walker = true;
       else ntype[d] = nrx > 0 ? react_decide(A, Rp, M, t, p, step, 0ull, 0ull) : t;
     }
@@ -3310,14 +3321,14 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
   mark(KT_SCATTER, true);
   if (c->any_rx)
     hipLaunchKernelGGL(k_scatter_react, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
-                       c->upos, c->ucell, c->urank, c->cstart, cur.type, cur.pid, cur.mol,
+                       c->upos, c->urank, c->cstart, cur.type, cur.pid, cur.mol,
                        nxt.pos, nxt.type, nxt.pid, nxt.mol, c->any_bonds ? c->slot_of : nullptr,
-                       (int)c->n_cells, nxt.type0, r);
+                       nxt.type0, r);
   else
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(c->n_total, 256)), dim3(256), 0, c->stream,
-                       (int)c->n_total, c->upos, c->ucell, c->urank, c->cstart, cur.type,
-                       cur.pid, cur.mol, nxt.pos, nxt.type, nxt.pid, nxt.mol,
-                       c->any_bonds ? c->slot_of : nullptr, (int)c->n_cells, c->d_status);
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
+                       c->d_sblocks, c->d_rep, c->cpos, (int)c->n_total, c->upos, c->urank,
+                       c->cstart, cur.type, cur.pid, cur.mol, nxt.pos, nxt.type, nxt.pid, nxt.mol,
+                       c->any_bonds ? c->slot_of : nullptr, c->d_status);
   mark(KT_SCATTER, false);
   if (c->any_rx) {
     mark(KT_REACT, true);
@@ -4024,9 +4035,9 @@ int rds_set_state(rds_ctx* c, const double* x, const int32_t* types, const int32
     hipLaunchKernelGGL(k_bin, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, c->d_rep,
                        c->d_rep_of, a.pos, c->ucell, c->urank, c->count, nullptr, c->cpos);
     launch_cell_starts(c, 0, 0);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, a.pos,
-                       c->ucell, c->urank, c->cstart, a.type, a.pid, a.mol, b.pos, b.type, b.pid,
-                       b.mol, c->slot_of, (int)c->n_cells, c->d_status);
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
+                       c->d_sblocks, c->d_rep, c->cpos, N, a.pos, c->urank, c->cstart, a.type,
+                       a.pid, a.mol, b.pos, b.type, b.pid, b.mol, c->slot_of, c->d_status);
     c->parity = 1;
   }
   long long s0 = step0;
