@@ -12,7 +12,7 @@ graphs of at most --steps steps.  Beside it:
   stratified subset of the same replicas, and the ratio batched / plain;
 * `roofline`: the fused force + integrate + bin kernel's average duration from
   HIP events on the engine's stream during an eager pass over the same state,
-  against its algorithmic HBM bytes (48 B per particle + 4 B per cell);
+  against its algorithmic HBM bytes (44 B per particle + 4 B per cell);
 * `cpu_baseline`: the CPU oracle in reference-faithful mode (Verlet list on the
   reference's grid, FP64, one core) on a bounded sample of the same replicas;
   `c1`: the one-species LJ system (N = 1,024, 2,000 steps, FP64) on the GPU
@@ -40,7 +40,7 @@ from pyrdsim_amd import ensemble, workloads  # noqa: E402
 from pyrdsim_amd.autotune import autotune  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-FORCE_BYTES_PER_PARTICLE = 48   # read pos 16 + type 4 + pid 4; write pos 16 + cell 4 + rank 4
+FORCE_BYTES_PER_PARTICLE = 44   # read pos 16 + type 4 + pid 4; write pos 16 + rank 4
 FORCE_BYTES_PER_CELL = 4        # cell starts
 # list mode: read pos 16 + type 4 + pid 4 + reference pos 16 + sorted cell 4 + list length 2,
 # write pos 16; plus 2 bytes per list entry

@@ -235,7 +235,9 @@ __device__ __forceinline__ void philox_normals(const DRep& R, int pid, long long
                           R.seed_lo, R.seed_hi);
   float ua = (float)((w.x >> 9) * 2u + 1u) * 0x1.0p-24f;  // odd / 2^24, in (0,1)
   float ub = (float)(w.y >> 8) * 0x1.0p-24f;              // [0,1)
-  float rr = sqrtf(-2.0f * logf(ua));
+  // -2 ln(ua) = -2 ln(2) log2(ua); ua >= 2^-24 is a normal float, so the
+  // hardware log2 and square root (about 1 ulp each) need no range handling
+  float rr = __builtin_amdgcn_sqrtf(-1.38629436f * __builtin_amdgcn_logf(ua));
   float s, c;
   sincospif(2.0f * ub, &s, &c);
   n0 = rr * c;
