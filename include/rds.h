@@ -146,7 +146,7 @@ typedef struct {
   int32_t list_capacity;
   /* 2 (0 means 2) or 3.  Three dimensions (pyrdsim.pyx:1159-1185,
    * 1999-2067 loop over N_dim) run the cell search (skin = 0) in FP64 or
-   * mixed precision; the analysis and tiling entry points refuse them. */
+   * mixed precision; the two-dimensional analysis entry points refuse them. */
   int32_t n_dim;
   /* harmonic angles (pyrdsim.pyx:1322-1407, 2052-2065) in definition order,
    * global particle indices, host arrays; n_angles may be 0.  angle_j is the
@@ -295,8 +295,10 @@ int rds_debug_fill_lds(rds_ctx* ctx, uint32_t pattern);
 
 /* Re-tiles a context (kernel work decomposition only; the state is kept):
  * tile_rows cell rows per tile (1, 2, 4) and about tile_particles particles
- * per tile row strip.  No counterpart in the reference; used by the
- * autotuner.  rds_get_tiling reports the current choice and LDS capacity. */
+ * per tile row strip (0: 56 in two dimensions, 48 in three, whose tiles are
+ * always 2 x 2 rows in y and z; tile_rows is then checked and reported, not
+ * used).  No counterpart in the reference; used by the autotuner.
+ * rds_get_tiling reports the current choice and LDS capacity. */
 int rds_set_tiling(rds_ctx* ctx, int32_t tile_rows, int32_t tile_particles);
 int rds_get_tiling(rds_ctx* ctx, int32_t* tile_rows, int32_t* tile_particles, int32_t* lds_cap);
 

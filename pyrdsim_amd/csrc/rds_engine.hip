@@ -2424,131 +2424,233 @@ __device__ __forceinline__ double3 disp3(const DRep& R, double4 a, double4 b) {
   return make_double3(dx, dy, dz);
 }
 
-// Pair forces (every term of the type pair, file order), walls and bonds of
-// the particle in slot s; then, in MODE_STEP, the Euler-Maruyama update
-// (pyrdsim.pyx:2244-2250), the boundary conditions (pyrdsim.pyx:2269-2292)
-// and the binning of the new position.  The 27-cell stencil is walked as nine
-// rows of three x-cells; a row is at most two contiguous slot ranges, each
-// with one periodic image shift (x_j + shift - x_i needs no minimum image:
-// a periodic axis with a stencil has at least three cells).  Candidates are
-// loaded four (general term table: two) at a time.  KM as in k_force: 0 = the general term table;
-// 1, 2 = one LJ term per type pair of at most two types, its constants in
-// registers (2: exponent n = 6).
-template <typename Real, bool MOL, int KM>
-__global__ __launch_bounds__(256) void k3_force(const StepArgs A, int mode, int pw) {
-  constexpr bool FAST = KM != 0;
-  const int4 B = A.sblocks[blockIdx.x];
-  const DRep& Rp = A.rep[B.x];
-  const DModel& M = A.models[Rp.model];
-  __shared__ Real s_fc[FAST ? 4 : 1][3];
-  if (FAST) {
-    if (threadIdx.x < 4) {
-      const int a = threadIdx.x >> 1, bb = threadIdx.x & 1, q = a * M.nt + bb;
-      Real v0 = 0, v1 = 0, v2 = 0;
-      if (a < M.nt && bb < M.nt && M.pcount[q] > 0) {
-        const DTerm& tm = M.terms[M.pfirst[q]];
-        v0 = TP<Real>::c_lj(tm);
-        v1 = TP<Real>::r0sq(tm);
-        v2 = TP<Real>::rc2(tm);
+// ---------------------------------------------------------------------------
+// Three dimensions, tiled: up to T3Y x T3Z home rows (y, z) of W cells in x.
+// The staged region is rows y0-1 .. y0+T3Y and layers z0-1 .. z0+T3Z, cells
+// x0-1 .. x0+W of each; staged row q = qz * (T3Y + 2) + qy.  Periodic images
+// are shifted by the box length at staging, so each staged row is at most
+// three contiguous slot segments, as in two dimensions.  A lane owns one home
+// particle and walks its nine row windows (three cells each) in LDS.
+// ---------------------------------------------------------------------------
+#define T3Y 2
+#define T3Z 2
+#define T3R ((T3Y + 2) * (T3Z + 2))
+#define MAXW3 16
+#define HMAX3 1024
+
+struct TileSh3 {
+  int cs[T3R][MAXW3 + 2];    // first slot of staged cell k (k = 0..W+1)
+  int off[T3R][MAXW3 + 3];   // staged offset of cell k within its row (prefix)
+  int row_base[T3R + 1];
+  int seg_slot[T3R][3], seg_n[T3R][3], seg_off[T3R][3];
+  double seg_sx[T3R][3];
+  double row_sy[T3R], row_sz[T3R];
+  int chp[T3Y * T3Z * MAXW3 + 1];  // home particles before home cell i = hr * W + k
+  unsigned char hkey[HMAX3];       // home number -> (hr << 4) | k, when <= HMAX3 of them
+  int total, nbatch;
+};
+
+// dynamic LDS of the tiled 3-D kernel: (x, y) as double2, z, type, molecule
+struct Staging3 {
+  double2* xy;
+  double* z;
+  int* ty;
+  int* mol;
+};
+__device__ __forceinline__ Staging3 staging3(char* smem, int n) {
+  Staging3 g;
+  g.xy = (double2*)smem;
+  g.z = (double*)(g.xy + n);
+  g.ty = (int*)(g.z + n);
+  g.mol = g.ty + n;
+  return g;
+}
+
+// Tile (replica | (ny-1) << 24 | (nz-1) << 26, y0, z0, x0 | W << 16).
+__device__ void tile_setup3(const DRep& Rp, const int* __restrict__ cstart, int y0, int z0,
+                            int x0, int W, int ny, int nz, TileSh3& sh) {
+  const int t = threadIdx.x, nthr = blockDim.x;
+  const int lx = Rp.lx, ly = Rp.ly, lz = Rp.lz;
+  const int ncs = T3R * (W + 2);
+  for (int q0 = t; q0 < ncs; q0 += 2 * nthr) {
+    int c[2], a[2], e[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int q = q0 + u * nthr;
+      const int r = q / (W + 2), k = q - r * (W + 2);
+      const int qy = r % (T3Y + 2), qz = r / (T3Y + 2);
+      // staged rows beyond a partial tile's last home row + 1 stay empty
+      bool ok = q < ncs && qy <= ny + 1 && qz <= nz + 1;
+      int y = y0 + qy - 1, z = z0 + qz - 1, x = x0 - 1 + k;
+      if (y < 0) { if (Rp.sy) y += ly; else ok = false; }
+      else if (y >= ly) { if (Rp.sy) y -= ly; else ok = false; }
+      if (z < 0) { if (Rp.sz) z += lz; else ok = false; }
+      else if (z >= lz) { if (Rp.sz) z -= lz; else ok = false; }
+      if (x < 0) { if (Rp.sx) x += lx; else ok = false; }
+      else if (x >= lx) { if (Rp.sx) x -= lx; else ok = false; }
+      c[u] = ok ? Rp.cbase + (z * ly + y) * lx + x : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      a[u] = c[u] >= 0 ? cstart[c[u]] : 0;
+      e[u] = c[u] >= 0 ? cstart[c[u] + 1] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int q = q0 + u * nthr;
+      if (q < ncs) {
+        const int r = q / (W + 2), k = q - r * (W + 2);
+        sh.cs[r][k] = a[u];
+        sh.off[r][k + 1] = e[u] - a[u];
       }
-      s_fc[threadIdx.x][0] = v0;
-      s_fc[threadIdx.x][1] = v1;
-      s_fc[threadIdx.x][2] = v2;
+    }
+  }
+  __syncthreads();
+  {
+    const int wv = t >> 6, lane = t & 63;
+    for (int r = wv; r < T3R; r += (nthr >> 6)) {
+      const int v = (lane < W + 2) ? sh.off[r][lane + 1] : 0;
+      int inc = v;
+#pragma unroll
+      for (int d = 1; d < 32; d <<= 1) {
+        const int o = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += o;
+      }
+      if (lane < W + 2) sh.off[r][lane + 1] = inc;
+      if (lane == 0) sh.off[r][0] = 0;
+    }
+  }
+  __syncthreads();
+  if (t < T3R) {
+    const int r = t;
+    const int qy = r % (T3Y + 2), qz = r / (T3Y + 2);
+    const int y = y0 + qy - 1, z = z0 + qz - 1;
+    double sy = 0.0, sz = 0.0;
+    if (y < 0) sy = -Rp.L[1];
+    else if (y >= ly) sy = Rp.L[1];
+    if (z < 0) sz = -Rp.L[2];
+    else if (z >= lz) sz = Rp.L[2];
+    sh.row_sy[r] = sy;
+    sh.row_sz[r] = sz;
+    for (int g = 0; g < 3; ++g) sh.seg_n[r][g] = 0;
+    // an invalid row has no particles (off[r][W + 2] == 0); its segments are
+    // empty whatever they say
+    if (x0 == 0 && Rp.sx) {
+      sh.seg_slot[r][0] = sh.cs[r][0];
+      sh.seg_n[r][0] = sh.off[r][1];
+      sh.seg_off[r][0] = 0;
+      sh.seg_sx[r][0] = -Rp.L[0];
+    }
+    const int ka = (x0 == 0) ? 1 : 0;
+    const int kb = (x0 + W >= lx) ? W : W + 1;
+    sh.seg_slot[r][1] = sh.cs[r][ka];
+    sh.seg_n[r][1] = sh.off[r][kb + 1] - sh.off[r][ka];
+    sh.seg_off[r][1] = sh.off[r][ka];
+    sh.seg_sx[r][1] = 0.0;
+    if (x0 + W >= lx && Rp.sx) {
+      sh.seg_slot[r][2] = sh.cs[r][W + 1];
+      sh.seg_n[r][2] = sh.off[r][W + 2] - sh.off[r][W + 1];
+      sh.seg_off[r][2] = sh.off[r][W + 1];
+      sh.seg_sx[r][2] = Rp.L[0];
+    }
+  }
+  if (t < 64) {
+    // first staged index of each staged row (prefix over the lanes)
+    const int rt = t < T3R ? sh.off[t][W + 2] : 0;
+    int ir = rt;
+#pragma unroll
+    for (int d = 1; d < 32; d <<= 1) {
+      const int a = __shfl_up(ir, d, 64);
+      if (t >= d) ir += a;
+    }
+    if (t <= T3R) sh.row_base[t] = ir - rt;
+    if (t == T3R) sh.total = ir;
+    // home cells: home row hr = (qz - 1) * ny + (qy - 1), cell k; one lane each
+    const int nc = ny * nz * W;
+    int hc = 0;
+    if (t < nc) {
+      const int hr = t / W, k = t - hr * W;
+      const int r = (hr / ny + 1) * (T3Y + 2) + (hr % ny + 1);
+      hc = sh.off[r][k + 2] - sh.off[r][k + 1];
+    }
+    int ih = hc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int a = __shfl_up(ih, d, 64);
+      if (t >= d) ih += a;
+    }
+    if (t < nc) sh.chp[t] = ih - hc;
+    if (t == nc - 1) {
+      sh.chp[nc] = ih;
+      sh.nbatch = (ih + (int)blockDim.x - 1) / (int)blockDim.x;
+    }
+  }
+  __syncthreads();
+  const int nc = ny * nz * W;
+  if (sh.chp[nc] <= HMAX3) {
+    for (int i = t; i < nc; i += nthr) {
+      const int hr = i / W, k = i - hr * W;
+      const unsigned char key = (unsigned char)((hr << 4) | k);
+      for (int g = sh.chp[i]; g < sh.chp[i + 1]; ++g) sh.hkey[g] = key;
     }
     __syncthreads();
   }
-  const int s = B.y + (int)threadIdx.x;
-  if (s >= B.z) return;
-  const double4 xi = A.pos4[s];
-  const int ti = A.type[s], p = A.pid[s];
-  const int mi = MOL ? A.mol[s] : 0;
-  const int tbase = ti * M.nt;
-  Real pa0 = 0, pb0 = 0, pc0 = 0, pa1 = 0, pb1 = 0, pc1 = 0;
-  if (FAST) {
-    const int a = ti == 1 ? 2 : 0;
-    pa0 = s_fc[a][0];
-    pb0 = s_fc[a][1];
-    pc0 = s_fc[a][2];
-    pa1 = s_fc[a + 1][0];
-    pb1 = s_fc[a + 1][1];
-    pc1 = s_fc[a + 1][2];
-  }
-  Real fx = 0, fy = 0, fz = 0;
-  auto pair = [&](int j, int tj, double4 pj, double shx, double shy, double shz) {
-    if (j == s || (MOL && A.mol[j] == mi)) return;
-    const Real rx = (Real)((pj.x + shx) - xi.x);
-    const Real ry = (Real)((pj.y + shy) - xi.y);
-    const Real rz = (Real)((pj.z + shz) - xi.z);
-    const Real r2 = rx * rx + ry * ry + rz * rz;
-    if (!(r2 > Real(0))) return;  // coincident particles: no direction
-    if (FAST) {
-      const bool m1 = tj == 1;
-      const Real pa = m1 ? pa1 : pa0, pb = m1 ? pb1 : pb0, pc = m1 ? pc1 : pc0;
-      const Real c = r2 <= pc ? lj_fast<Real, KM>(r2, pa, pb, pw) : Real(0);
-      fx += c * rx;
-      fy += c * ry;
-      fz += c * rz;
-    } else {
-      const int pq = tbase + tj;
-      const int q0 = M.pfirst[pq], qn = M.pcount[pq];
-      for (int q = q0; q < q0 + qn; ++q) {
-        const DTerm& tm = M.terms[q];
-        if (!(r2 <= TP<Real>::rc2(tm))) continue;
-        const Real c = pair_coef<Real>(tm, r2);
-        fx += c * rx;
-        fy += c * ry;
-        fz += c * rz;
-      }
-    }
-  };
-  constexpr int NB = FAST ? 4 : 2;  // candidates whose loads are in flight together
-  auto range = [&](int a, int b, double shx, double shy, double shz) {
-    for (int j0 = a; j0 < b; j0 += NB) {
-      int tj[NB];
-      double4 pj[NB];
+}
+
+// Staged indices [p0, p0 + cap) into LDS (types shifted left by TSH).
+template <bool MOL, int TSH>
+__device__ void tile_stage3(const TileSh3& sh, int p0, int cap, const double4* __restrict__ pos,
+                            const int* __restrict__ type, const int* __restrict__ mol,
+                            const Staging3& G) {
+  const int t = threadIdx.x, nthr = blockDim.x;
+  const int p1 = min(p0 + cap, sh.total);
+  for (int pb = p0 + t; pb < p1; pb += STAGE_U * nthr) {
+    int src[STAGE_U], rg[STAGE_U];
 #pragma unroll
-      for (int v = 0; v < NB; ++v) {
-        const int j = min(j0 + v, b - 1);
-        tj[v] = A.type[j];
-        pj[v] = A.pos4[j];
-      }
+    for (int u = 0; u < STAGE_U; ++u) {
+      const int p = min(pb + u * nthr, p1 - 1);
+      // row: the last with row_base <= p (binary search over the T3R rows)
+      int r = 0;
 #pragma unroll
-      for (int v = 0; v < NB; ++v)
-        if (j0 + v < b) pair(j0 + v, tj[v], pj[v], shx, shy, shz);
+      for (int step = T3R / 2; step >= 1; step >>= 1)
+        if (sh.row_base[r + step] <= p) r += step;
+      const int loc = p - sh.row_base[r];
+      int g = 0;
+      if (sh.seg_n[r][1] > 0 && loc >= sh.seg_off[r][1]) g = 1;
+      if (sh.seg_n[r][2] > 0 && loc >= sh.seg_off[r][2]) g = 2;
+      src[u] = sh.seg_slot[r][g] + (loc - sh.seg_off[r][g]);
+      rg[u] = r * 3 + g;
     }
-  };
-  const int cx = cell_coord(xi.x, Rp.cl[0], Rp.lx), cy = cell_coord(xi.y, Rp.cl[1], Rp.ly);
-  const int cz = cell_coord(xi.z, Rp.cl[2], Rp.lz);
-  const int lx = Rp.lx;
-#pragma unroll 1
-  for (int r = 0; r < 9; ++r) {
-    int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
-    double shz = 0.0, shy = 0.0;
-    if (z < 0 || z >= Rp.lz) {
-      if (!Rp.sz) continue;
-      shz = z < 0 ? -Rp.L[2] : Rp.L[2];
-      z = z < 0 ? z + Rp.lz : z - Rp.lz;
+    double4 q[STAGE_U];
+    int ty[STAGE_U], mo[STAGE_U];
+#pragma unroll
+    for (int u = 0; u < STAGE_U; ++u) {
+      q[u] = pos[src[u]];
+      ty[u] = type[src[u]];
+      mo[u] = MOL ? mol[src[u]] : 0;
     }
-    if (y < 0 || y >= Rp.ly) {
-      if (!Rp.sy) continue;
-      shy = y < 0 ? -Rp.L[1] : Rp.L[1];
-      y = y < 0 ? y + Rp.ly : y - Rp.ly;
-    }
-    const int rowc = Rp.cbase + (z * Rp.ly + y) * lx;
-    if (!Rp.sx) {
-      range(A.cstart[rowc + max(cx - 1, 0)], A.cstart[rowc + min(cx + 1, lx - 1) + 1], 0.0, shy,
-            shz);
-    } else if (cx == 0) {
-      range(A.cstart[rowc + lx - 1], A.cstart[rowc + lx], -Rp.L[0], shy, shz);
-      range(A.cstart[rowc], A.cstart[rowc + 2], 0.0, shy, shz);
-    } else if (cx == lx - 1) {
-      range(A.cstart[rowc + cx - 1], A.cstart[rowc + lx], 0.0, shy, shz);
-      range(A.cstart[rowc], A.cstart[rowc + 1], Rp.L[0], shy, shz);
-    } else {
-      range(A.cstart[rowc + cx - 1], A.cstart[rowc + cx + 2], 0.0, shy, shz);
+#pragma unroll
+    for (int u = 0; u < STAGE_U; ++u) {
+      const int p = pb + u * nthr;
+      if (p >= p1) break;
+      const int r = rg[u] / 3, g = rg[u] - 3 * r;
+      G.xy[p - p0] = make_double2(q[u].x + sh.seg_sx[r][g], q[u].y + sh.row_sy[r]);
+      G.z[p - p0] = q[u].z + sh.row_sz[r];
+      G.ty[p - p0] = ty[u] << TSH;
+      if (MOL) G.mol[p - p0] = mo[u];
     }
   }
-  double F[3] = {(double)fx, (double)fy, (double)fz};
+}
+
+// The integrator epilogue of a three-dimensional force kernel: walls, bond
+// forces, then either the force (MODE_FORCE) or the Euler-Maruyama update
+// (pyrdsim.pyx:2244-2250), the boundary conditions (pyrdsim.pyx:2269-2292),
+// the new cell and the rank in it.
+template <typename Real>
+__device__ __forceinline__ void k3_finish(const StepArgs& A, const DRep& Rp, const DModel& M,
+                                          int mode, int s, int p, int ti, double4 xi,
+                                          double fx, double fy, double fz) {
+  double F[3] = {fx, fy, fz};
   const double X[3] = {xi.x, xi.y, xi.z};
   for (int q = 0; q < M.nwalls; ++q) {
     const DWall& wl = M.walls[q];
@@ -2621,6 +2723,209 @@ __global__ __launch_bounds__(256) void k3_force(const StepArgs A, int mode, int 
   A.upos4[s] = make_double4(v[0], v[1], v[2], 0.0);
   A.ucell[s] = c;
   A.urank[s] = atomicAdd(&A.count[c], 1);
+}
+
+// Three-dimensional force kernel: pair forces (every term of the type pair,
+// file order) over the 27-cell stencil, walked as nine staged rows of three
+// x-cells in LDS (x_j + shift - x_i needs no minimum image: a periodic axis
+// with a stencil has at least three cells); walls and bonds; then, in
+// MODE_STEP, the update, the boundary conditions and the binning (k3_finish).
+// Differences in FP64, pair arithmetic and sums in Real (FP32 or FP64).  KM
+// as in k_force: 0 = the general term table; 1, 2 = one LJ term per type pair
+// of at most two types (2: exponent n = 6).
+template <typename Real, bool MOL, int KM>
+__global__ __launch_bounds__(256) void k3_tile(const StepArgs A, int mode, int pw) {
+  constexpr bool FAST = KM != 0;
+  constexpr int TSH = (FAST && sizeof(Real) == 4) ? 4 : 0;
+  constexpr bool LJ6 = TSH && KM == 2;
+  extern __shared__ __align__(16) char smem[];
+  const int CAPT = A.cap;
+  const Staging3 G = staging3(smem, CAPT + 1);
+  __shared__ TileSh3 sh;
+  __shared__ int s_pfirst[FAST ? 1 : MAXT * MAXT], s_pcount[FAST ? 1 : MAXT * MAXT];
+  __shared__ DTerm s_term[FAST ? 1 : MAXTERMS];
+  __shared__ Real s_fc[FAST ? 4 : 1][3];
+  __shared__ __align__(16) float4 s_fc4[TSH ? 4 : 1];
+
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int tile = xcd_tile(blockIdx.x, A.n_tiles);
+  const int4 T = A.tiles[tile];
+  const DRep& Rp = A.rep[T.x & 0xffffff];
+  const int ny = ((T.x >> 24) & 3) + 1, nz = ((T.x >> 26) & 3) + 1;
+  const int y0 = T.y, z0 = T.z, x0 = T.w & 0xffff, W = T.w >> 16;
+  const DModel& M = A.models[Rp.model];
+  const int nt = M.nt;
+
+  if (!FAST) {
+    for (int q = t; q < nt * nt; q += blockDim.x) {
+      s_pfirst[q] = M.pfirst[q];
+      s_pcount[q] = M.pcount[q];
+    }
+    for (int q = t; q < M.nterms; q += blockDim.x) s_term[q] = M.terms[q];
+  }
+  if (FAST && t < 4) {
+    const int a = t >> 1, bb = t & 1, q = a * nt + bb;
+    Real v0 = 0, v1 = 0, v2 = 0;
+    double A6 = 0.0, B6 = 0.0;
+    if (a < nt && bb < nt && M.pcount[q] > 0) {
+      const DTerm& tm = M.terms[M.pfirst[q]];
+      v0 = TP<Real>::c_lj(tm);
+      v1 = TP<Real>::r0sq(tm);
+      v2 = TP<Real>::rc2(tm);
+      const double r6 = tm.r0sq * tm.r0sq * tm.r0sq;
+      B6 = tm.c_lj * r6;
+      A6 = B6 * r6;
+    }
+    s_fc[t][0] = v0;
+    s_fc[t][1] = v1;
+    s_fc[t][2] = v2;
+    if (TSH)
+      s_fc4[t] = LJ6 ? make_float4((float)A6, (float)B6, (float)v2, 0.f)
+                     : make_float4((float)v0, (float)v1, (float)v2, 0.f);
+  }
+  tile_setup3(Rp, A.cstart, y0, z0, x0, W, ny, nz, sh);
+  const int total = sh.total;
+  const bool resident = total <= CAPT;
+  if (resident) {
+    tile_stage3<MOL, TSH>(sh, 0, CAPT, A.pos4, A.type, A.mol, G);
+    if (t == 0) {
+      G.xy[total] = make_double2(0.0, 0.0);
+      G.z[total] = 0.0;
+      G.ty[total] = 0;
+      if (MOL) G.mol[total] = -1;
+    }
+    __syncthreads();
+  }
+  const int nc = ny * nz * W;
+  const int nh = sh.chp[nc];
+  for (int b = 0; b < sh.nbatch; ++b) {
+    // home particle g: its home cell from the table (or the largest i with
+    // chp[i] <= g), the slot from the cell's first slot
+    const int g = b * (int)blockDim.x + t;
+    const bool valid = g < nh;
+    int hr = 0, k = 0;
+    if (valid) {
+      if (nh <= HMAX3) {
+        const int key = sh.hkey[g];
+        hr = key >> 4;
+        k = key & 15;
+      } else {
+        int lo = 0, hi = nc;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (sh.chp[mid] <= g) lo = mid;
+          else hi = mid;
+        }
+        hr = lo / W;
+        k = lo - hr * W;
+      }
+    }
+    const int rh = (hr / ny + 1) * (T3Y + 2) + (hr % ny + 1);  // staged row of the home row
+    const int s = valid ? sh.cs[rh][k + 1] + (g - sh.chp[hr * W + k]) : 0;
+    double4 xi = make_double4(0.0, 0.0, 0.0, 0.0);
+    int ti = 0, p = 0, mi = 0;
+    if (valid) {
+      xi = A.pos4[s];
+      ti = A.type[s];
+      p = A.pid[s];
+      if (MOL) mi = A.mol[s];
+    }
+    Real pa0 = 0, pb0 = 0, pc0 = 0, pa1 = 0, pb1 = 0, pc1 = 0;
+    if (FAST && !TSH) {
+      const int a = ti == 1 ? 2 : 0;
+      pa0 = s_fc[a][0];
+      pb0 = s_fc[a][1];
+      pc0 = s_fc[a][2];
+      pa1 = s_fc[a + 1][0];
+      pb1 = s_fc[a + 1][1];
+      pc1 = s_fc[a + 1][2];
+    }
+    const char* fc_row = reinterpret_cast<const char*>(s_fc4) + ti * 32;
+    const int tbase = ti * nt;
+    Real fx = 0, fy = 0, fz = 0;
+    // one candidate at difference (dx, dy, dz), staged type tj (shifted), counted when `live`
+    auto pairv = [&](Real dx, Real dy, Real dz, int tj, int mj, bool live) {
+      const Real r2 = dx * dx + dy * dy + dz * dz;
+      if (FAST) {
+        Real pa, pb, pc;
+        if (TSH) {
+          const float4 kc = *reinterpret_cast<const float4*>(fc_row + tj);
+          pa = (Real)kc.x;
+          pb = (Real)kc.y;
+          pc = (Real)kc.z;
+        } else {
+          const bool m1 = tj == 1;
+          pa = m1 ? pa1 : pa0;
+          pb = m1 ? pb1 : pb0;
+          pc = m1 ? pc1 : pc0;
+        }
+        Real c;
+        if constexpr (LJ6) c = lj_fast6(r2, pa, pb);
+        else c = lj_fast<Real, KM>(r2, pa, pb, pw);
+        // r2 > 0 excludes the particle itself (and coincident ones)
+        const bool take = MOL ? (mj != mi && r2 > Real(0) && r2 <= pc) : in_range(r2, pc);
+        c = (live && take) ? c : Real(0);
+        fx += c * dx;
+        fy += c * dy;
+        fz += c * dz;
+      } else {
+        const bool other = MOL ? (mj != mi && r2 > Real(0)) : (r2 > Real(0));
+        const int pq = tbase + tj;
+        const int q0 = s_pfirst[pq], qn = live ? s_pcount[pq] : 0;
+        for (int q = q0; q < q0 + qn; ++q) {
+          const DTerm& tm = s_term[q];
+          Real c = pair_coef<Real>(tm, r2);
+          c = (other && r2 <= TP<Real>::rc2(tm)) ? c : Real(0);
+          fx += c * dx;
+          fy += c * dy;
+          fz += c * dz;
+        }
+      }
+    };
+    for (int p0 = 0; p0 < total; p0 += CAPT) {
+      if (!resident) {
+        __syncthreads();
+        tile_stage3<MOL, TSH>(sh, p0, CAPT, A.pos4, A.type, A.mol, G);
+        if (t == 0) {
+          const int e = min(total - p0, CAPT);
+          G.xy[e] = make_double2(0.0, 0.0);
+          G.z[e] = 0.0;
+          G.ty[e] = 0;
+          if (MOL) G.mol[e] = -1;
+        }
+        __syncthreads();
+      }
+      if (!valid) continue;
+      // the nine row windows (cells k .. k + 2 of the staged rows around the
+      // home row, clipped to this LDS round), two consecutive staged
+      // particles per trip; past a window's end the second candidate is the
+      // next staged particle or the sentinel, finite either way, and masked
+#pragma unroll 1
+      for (int d = 0; d < 9; ++d) {
+        const int rr = rh + (d / 3 - 1) * (T3Y + 2) + (d % 3 - 1);
+        const int base = sh.row_base[rr];
+        const int lo = max(base + sh.off[rr][k], p0) - p0;
+        const int hi = min(base + sh.off[rr][k + 3], p0 + CAPT) - p0;
+        const double2* pq = G.xy + lo;
+        const double2* const pe = G.xy + hi;
+        const double2* const pl = pe - 1;
+        const double* zq = G.z + lo;
+        const int* tq = G.ty + lo;
+        const int* mq = G.mol + lo;
+        for (; pq < pe; pq += 2, zq += 2, tq += 2, mq += MOL ? 2 : 0) {
+          const double2 q0 = pq[0], q1 = pq[1];
+          const double z0v = zq[0], z1v = zq[1];
+          const int t0 = tq[0], t1 = tq[1];
+          const int m0 = MOL ? mq[0] : 0, m1 = MOL ? mq[1] : 0;
+          pairv((Real)(q0.x - xi.x), (Real)(q0.y - xi.y), (Real)(z0v - xi.z), t0, m0, true);
+          pairv((Real)(q1.x - xi.x), (Real)(q1.y - xi.y), (Real)(z1v - xi.z), t1, m1, pq < pl);
+        }
+      }
+    }
+    if (!valid) continue;
+    k3_finish<Real>(A, Rp, M, mode, s, p, ti, xi, (double)fx, (double)fy, (double)fz);
+  }
 }
 
 // Harmonic bonds and angles in three dimensions (pyrdsim.pyx:1303-1407,
@@ -3212,13 +3517,14 @@ void launch_bonds(rds_ctx* c, const Buf& cur) {
 
 template <typename Real, bool MOL>
 void launch_force3_m(rds_ctx* c, const StepArgs& a, int mode) {
-  const dim3 g((unsigned)c->n_sblocks), b(256);
+  const dim3 g((unsigned)c->n_tiles), b(256);
+  const size_t sm = c->smem_bytes;
   if (!c->fast)
-    hipLaunchKernelGGL((k3_force<Real, MOL, 0>), g, b, 0, c->stream, a, mode, c->pw);
+    hipLaunchKernelGGL((k3_tile<Real, MOL, 0>), g, b, sm, c->stream, a, mode, c->pw);
   else if (c->pw == 3)
-    hipLaunchKernelGGL((k3_force<Real, MOL, 2>), g, b, 0, c->stream, a, mode, c->pw);
+    hipLaunchKernelGGL((k3_tile<Real, MOL, 2>), g, b, sm, c->stream, a, mode, c->pw);
   else
-    hipLaunchKernelGGL((k3_force<Real, MOL, 1>), g, b, 0, c->stream, a, mode, c->pw);
+    hipLaunchKernelGGL((k3_tile<Real, MOL, 1>), g, b, sm, c->stream, a, mode, c->pw);
 }
 
 void launch_force3(rds_ctx* c, const StepArgs& a, int mode) {
@@ -3546,6 +3852,51 @@ bool build_tiles(rds_ctx* c, int rows, int target) {
   return true;
 }
 
+// Three-dimensional tiles: T3Y x T3Z home rows (y, z) x W cells, W chosen so
+// that a row strip holds about `target` particles at the replica's mean
+// occupancy; the LDS round capacity: 1.25x the mean staged count of the
+// densest tile shape, at least 256, at most what 64 KB of dynamic LDS holds
+// (denser tiles stage in rounds).
+bool build_tiles3(rds_ctx* c, int target) {
+  if (target <= 0) target = 48;
+  c->h_tiles.clear();
+  double staged_worst = 0.0;
+  int stride = std::max(1, (int)std::lround(0.618 * c->n_rep));
+  while (std::gcd(stride, c->n_rep) != 1) ++stride;
+  for (int q = 0; q < c->n_rep; ++q) {
+    const int r = (int)(((long long)q * stride) % c->n_rep);
+    const DRep& d = c->h_rep[r];
+    const double occ = (double)d.n / ((double)d.lx * d.ly * d.lz);
+    int W = (int)std::floor((double)target / std::max(occ, 1e-9));
+    W = std::max(1, std::min(W, std::min(MAXW3, d.lx)));
+    const int nstrip = (d.lx + W - 1) / W;
+    const int Wq = d.lx / nstrip, Wr = d.lx % nstrip;
+    staged_worst = std::max(staged_worst, occ * T3R * (Wq + (Wr ? 1 : 0) + 2));
+    for (int z = 0; z < d.lz; z += T3Z) {
+      const int nz = std::min(T3Z, d.lz - z);
+      for (int y = 0; y < d.ly; y += T3Y) {
+        const int ny = std::min(T3Y, d.ly - y);
+        int x0 = 0;
+        for (int k = 0; k < nstrip; ++k) {
+          const int wk = Wq + (k < Wr ? 1 : 0);
+          c->h_tiles.push_back(make_int4(r | ((ny - 1) << 24) | ((nz - 1) << 26), y, z,
+                                         x0 | (wk << 16)));
+          x0 += wk;
+        }
+      }
+    }
+  }
+  c->n_tiles = (int64_t)c->h_tiles.size();
+  const int per = 16 + 8 + 4 + (c->has_mol ? 4 : 0);
+  int cap = (int)std::ceil(1.25 * staged_worst / 64.0) * 64;
+  cap = std::max(cap, 256);
+  cap = std::min(cap, (65536 - 256) / per / 64 * 64);
+  c->cap = cap;
+  // one sentinel slot beyond the capacity
+  c->smem_bytes = (size_t)(cap + 1) * per;
+  return true;
+}
+
 int upload_tiles(rds_ctx* c) {
   if (c->d_tiles) hipFree(c->d_tiles);
   c->d_tiles = nullptr;
@@ -3563,11 +3914,18 @@ int rds_abi_version(void) { return RDS_ABI_VERSION; }
 
 int rds_set_tiling(rds_ctx* c, int32_t tile_rows, int32_t tile_particles) {
   if (!c) return fail(RDS_E_ARG, "null context");
-  if (c->ndim != 2) return fail(RDS_E_ARG, "three-dimensional contexts have no force tiles");
   HIP_TRY(hipStreamSynchronize(c->stream));
-  if (!build_tiles(c, tile_rows, tile_particles)) return RDS_E_ARG;
+  if (c->ndim == 3) {
+    // tiles of T3Y x T3Z rows; tile_rows is accepted and kept, not used
+    if (tile_rows != 1 && tile_rows != 2 && tile_rows != 4)
+      return fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4");
+    c->rows = tile_rows;
+    build_tiles3(c, tile_particles);
+  } else if (!build_tiles(c, tile_rows, tile_particles)) {
+    return RDS_E_ARG;
+  }
   RDS_TRY(upload_tiles(c));
-  c->target = tile_particles > 0 ? tile_particles : 56;
+  c->target = tile_particles > 0 ? tile_particles : (c->ndim == 3 ? 48 : 56);
   if (c->loaded && c->list) {
     // lists hold staged indices of the old tiles
     force_rebuild(c);
@@ -3682,7 +4040,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   c->rows = cfg->tile_rows > 0 ? cfg->tile_rows : 1;
   if (c->rows != 1 && c->rows != 2 && c->rows != 4)
     return bail(fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4"));
-  c->target = cfg->tile_particles > 0 ? cfg->tile_particles : 56;
+  c->target = cfg->tile_particles > 0 ? cfg->tile_particles : (c->ndim == 3 ? 48 : 56);
   if (cfg->n_replicas >= (1 << 24)) return bail(fail(RDS_E_ARG, "too many replicas"));
   const int spc = cfg->cells_per_cutoff != 0 ? std::abs(cfg->cells_per_cutoff) : 1;
   if (spc != 1) return bail(fail(RDS_E_ARG, "only cells_per_cutoff = 1 (3x3 stencil) is built"));
@@ -3800,8 +4158,9 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   }
   c->n_total = pbase;
   c->n_cells = cbase;
-  // force tiles (two dimensions; the three-dimensional kernels walk per slot)
+  // force tiles
   if (c->ndim == 2 && !build_tiles(c, c->rows, (int)target)) return bail(RDS_E_ARG);
+  if (c->ndim == 3) build_tiles3(c, cfg->tile_particles);
 
   HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   if ((rc = scratch_ptr(c->device, &c->static_labels)) != RDS_OK) return bail(rc);

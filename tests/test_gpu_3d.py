@@ -199,8 +199,8 @@ def test_wall3_run_stays_in_the_box(gpu):
 
 def test_3d_refuses_what_it_does_not_build(gpu):
     """Three dimensions run the cell search in FP64 or mixed precision; the
-    list mode, compensated positions, force tiles and the 2-D analysis entry
-    points are refused with an error, not run."""
+    list mode, compensated positions and the 2-D analysis entry points are
+    refused with an error, not run."""
     s = small('lj3', n=1000)
     with pytest.raises(RuntimeError, match='three dimensions'):
         engine_for(s, skin=0.3)
@@ -211,12 +211,31 @@ def test_3d_refuses_what_it_does_not_build(gpu):
         eng.neighbor_counts(1.0)
     with pytest.raises(RuntimeError, match='two-dimensional'):
         eng.cluster_labels()
-    with pytest.raises(RuntimeError, match='force tiles'):
-        eng.set_tiling(2, 56)
     # a 2-D box handed to a 3-D engine is refused by make_replica's caller
     from pyrdsim_amd.engine import make_replica
     with pytest.raises(RuntimeError):
         make_replica(0, 10, [5.0], [1], 1e-3, [1.0], [], 1)
+
+
+@pytest.mark.parametrize('name', ['lj3', 'patchy3'])
+def test_3d_retiling_keeps_state_and_forces(gpu, name):
+    """Re-tiling a 3-D context (tiles of 2 x 2 cell rows in y and z, strips
+    of about tile_particles particles in x, from one cell to the whole row)
+    changes only the work decomposition: same state, same FP64 forces up to
+    summation order."""
+    s = small(name, n_mol=400) if name == 'patchy3' else small(name, n=3000)
+    eng = engine_for(s, precision='fp64', seed=4)
+    eng.run(5)
+    x0, t0 = eng.state()
+    f0 = eng.forces()
+    scale = np.abs(f0).max() + 1.0
+    for per in (1, 200, 4000, 0):
+        eng.set_tiling(1, per)
+        x1, t1 = eng.state()
+        assert np.array_equal(x0, x1) and np.array_equal(t0, t1)
+        assert np.abs(eng.forces() - f0).max() <= 1e-12 * scale, per
+    eng.run(3)
+    assert np.isfinite(eng.state()[0]).all()
 
 
 def test_facade_simulates_3d(gpu, tmp_path):
