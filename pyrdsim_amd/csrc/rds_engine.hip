@@ -1560,6 +1560,43 @@ __device__ __forceinline__ int react_decide(const StepArgs& A, const DRep& Rp, c
   return M.rx[chosen].product;
 }
 
+// True when particle p (type ti) cannot react this step whatever its
+// neighbour counts, each at most cmax: with every catalysed count at cmax and
+// every inhibitor count at 0 the total rate is at its largest (rx_rate's sum
+// grows with each catalysed count and shrinks with each inhibitor count when
+// no rate is negative), and even that gives a probability below u1 -- the
+// same u1 react_decide draws -- with a relative margin of 1e-9, far above the
+// rounding of either sum.  Then react_decide would return ti.  Nothing is
+// concluded for a negative rate or an injected draw out of range.
+__device__ __forceinline__ bool certainly_inert(const StepArgs& A, const DRep& Rp,
+                                                const DModel& M, int ti, int p, long long step,
+                                                int cmax) {
+  double kmax = 0.0;
+#pragma unroll 1
+  for (int q = 0; q < M.nrx; ++q) {
+    const DRx& rx = M.rx[q];
+    if (rx.reactant != ti) continue;
+    const double rate = Rp.rate[q];
+    if (!(rate >= 0.0)) return false;
+    if (rx.kind == RDS_RX_CATALYZED) kmax += rate * (double)cmax;
+    else if (rx.kind == RDS_RX_LINEAR) kmax += rate;
+    else {
+      if (!(rx.rate_decrease >= 0.0)) return false;
+      kmax += rate;
+    }
+  }
+  if (!(kmax > 0.0)) return true;  // every rate 0: react_decide returns ti
+  double u1, u2;
+  if (A.rng == RDS_RNG_INJECTED) {
+    const long long o = step - A.inj_first;
+    if (o < 0 || o >= A.inj_count) return false;
+    u1 = A.inj_u1[o * A.n_total + p];
+  } else {
+    philox_uniforms(Rp, p - Rp.pbase, step, u1, u2);
+  }
+  return !(Rp.dt * kmax * (1.0 + 1e-9) > u1);
+}
+
 // List mode, first half of the reaction step, per slot block: every particle
 // whose rates need no neighbours is decided here (pyrdsim.pyx:2103-2222); the
 // others are listed for k_react_walk, as k_scatter_react lists them in cell
@@ -1684,6 +1721,18 @@ __global__ __launch_bounds__(256, 6) void k_react_walk(const StepArgs A, int n_b
       ra1[r] = ok ? A.cstart[i1] : ra0[r];
       rb0[r] = A.cstart[i2];
       rb1[r] = ok ? A.cstart[i3] : rb0[r];
+    }
+    // no neighbour count can exceed the particles of the stencil: when even
+    // that many could not make the particle react, it keeps its type and
+    // nothing more is read
+    {
+      int pop = 0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) pop += (ra1[r] - ra0[r]) + (rb1[r] - rb0[r]);
+      if (certainly_inert(A, Rp, M, ti, p, step, pop)) {
+        A.type_out[s] = ti;
+        continue;
+      }
     }
 #pragma unroll 1
     for (int r = 0; r < 3; ++r) {

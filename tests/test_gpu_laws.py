@@ -90,11 +90,18 @@ def test_all_pair_laws_forces(gpu, precision, rtol, skin):
 
 
 @pytest.mark.parametrize('skin', [0.0, 0.2])
-def test_all_reaction_kinds_trajectory(gpu, skin):
+@pytest.mark.parametrize('u1_scale', [1e-3, 3e-3])
+def test_all_reaction_kinds_trajectory(gpu, skin, u1_scale):
+    """Every reaction kind against the oracle, event for event.  With u1 up
+    to 3e-3 the draws of the catalysed (p = dt * 0.5 * count) and inhibited
+    (p = dt * (2 - 0.5 * count)) reactants fall in all three of: below p (the
+    particle reacts), between p and the bound from its stencil's particle
+    count (it is walked and does not react), above the bound (it is not
+    walked: no count could make it react)."""
     s = _system(seed=32)
     steps = 40
     normals, u1, u2 = _injected(len(s['x']), steps, 33)
-    u1 *= 1e-3
+    u1 *= u1_scale
     eng = engine_for(s, precision='fp64', rng='injected', skin=skin)
     eng.set_injected(normals, u1, u2, first_step=1)
     o = oracle_for(s, rng='injected')
