@@ -196,9 +196,12 @@ struct StepArgs {
   const int* boff;      // bond CSR by pid (null without bonds)
   const int* bpart;
   // cell mode: slots whose reaction rates need a neighbour count, listed by
-  // slot block (sblocks): block b's in wl[256 b ..), their number in wl_n[b]
-  int* wl;
+  // slot block (sblocks): block b's in wl[256 b ..) as (slot, cell, type,
+  // pid), their number in wl_n[b]; the cell is the new one in cell mode and
+  // that of the last rebuild in list mode
+  int4* wl;
   int* wl_n;
+  const int4* wblocks;  // (slot block, replica, model) of replicas whose model has a walk
   // three dimensions: positions {x, y, z, 0}, next positions, bond forces
   const double4* pos4;
   double4* upos4;
@@ -1620,7 +1623,9 @@ __global__ __launch_bounds__(256) void k_react_list(const StepArgs A) {
   __syncthreads();
   int base = 0;
   for (int w = 0; w < wv; ++w) base += s_wcnt[w];
-  if (walker) A.wl[blockIdx.x * 256 + base + __popcll(m & ((1ull << lane) - 1ull))] = s;
+  if (walker)
+    A.wl[blockIdx.x * 256 + base + __popcll(m & ((1ull << lane) - 1ull))] =
+        make_int4(s, A.scell[s], A.type[s], A.pid[s]);
   if (threadIdx.x == 0) A.wl_n[blockIdx.x] = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
 }
 
@@ -1635,22 +1640,30 @@ __global__ __launch_bounds__(256) void k_react_list(const StepArgs A) {
 #define WALK_BATCH 12
 template <bool MOL>
 __global__ __launch_bounds__(256, 6) void k_react_walk(const StepArgs A, int n_blocks) {
-  const int sb = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (sb >= n_blocks) return;
+  // one wave per slot block of a replica whose model has a walk: (slot
+  // block, replica, model); the block's list entries (slot, cell, type, pid)
+  const int wb = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (wb >= n_blocks) return;
+  const int4 WB = A.wblocks[wb];
+  const int sb = WB.x;
+  const DRep& Rp = A.rep[WB.y];
+  const DModel& M = A.models[WB.z];
   const int n = A.wl_n[sb];
+  // the lane's first entry is loaded before the count is known (the
+  // block's 256 entries are allocated; one past the count is not used)
+  const int4 e0 = A.wl[sb * 256 + (int)(threadIdx.x & 63)];
   if (n == 0) return;
-  const int4 B = A.sblocks[sb];
-  const DRep& Rp = A.rep[B.x];
-  const DModel& M = A.models[Rp.model];
   const int nrx = M.nrx;
   const long long step = *A.step;
   const int lx = Rp.lx, ly = Rp.ly, cbase = Rp.cbase, sx = Rp.sx, sy = Rp.sy;
   const int px = Rp.px, py = Rp.py;
-  const double L0 = Rp.L[0], L1 = Rp.L[1], cl0 = Rp.cl[0], cl1 = Rp.cl[1];
+  const double L0 = Rp.L[0], L1 = Rp.L[1];
   for (int i = threadIdx.x & 63; i < n; i += 64) {
-    const int s = A.wl[sb * 256 + i];
-    const int ti = A.type[s];
-    const int p = A.pid[s];
+    // the slot's cell: in list mode the cell of the last rebuild (cells are
+    // then at least range + skin wide, so its stencil holds every particle
+    // within r0; distances take the minimum image); in cell mode the new cell
+    const int4 e = i < 64 ? e0 : A.wl[sb * 256 + i];
+    const int s = e.x, cell = e.y - cbase, ti = e.z, p = e.w;
     const double2 xi = rd_pos(A.pos, s, A.cpos);
     const int mi = MOL ? A.mol[s] : 0;
     // the slot's neighbour-dependent reactions: the first two in registers
@@ -1676,18 +1689,7 @@ __global__ __launch_bounds__(256, 6) void k_react_walk(const StepArgs A, int n_b
     if (qa >= 0) { lo_a = M.rx[qa].r0sq_lo; hi_a = M.rx[qa].r0sq_hi; r0_a = M.rx[qa].r0; }
     if (qb >= 0) { lo_b = M.rx[qb].r0sq_lo; hi_b = M.rx[qb].r0sq_hi; r0_b = M.rx[qb].r0; }
     unsigned long long c_lo = 0ull, c_hi = 0ull;  // packed counts (cnt_of)
-    // the slot's cell: in list mode the cell of the last rebuild (cells are
-    // then at least range + skin wide, so its stencil holds every particle
-    // within r0; distances take the minimum image)
-    int cx, cy;
-    if (A.scell) {
-      const int c = A.scell[s] - cbase;
-      cy = c / lx;
-      cx = c - cy * lx;
-    } else {
-      cx = cell_coord(xi.x, cl0, lx);
-      cy = cell_coord(xi.y, cl1, ly);
-    }
+    const int cy = cell / lx, cx = cell - cy * lx;
     // the stencil rows' cells cx-1 .. cx+1, each as at most two contiguous
     // slot ranges [ra0, ra1) and [rb0, rb1); a row outside a reflecting wall
     // is empty.  The cell-start loads of all three rows are issued together.
@@ -2052,10 +2054,11 @@ __global__ __launch_bounds__(256) void k_scatter_react(
   const long long step = *A.step;
   const int s = B.y + (int)threadIdx.x;
   bool walker = false;
-  int d = -1;
+  int d = -1, c = 0, tw = 0, pw = 0;  // a walker's slot, cell, type, pid
   if (s < B.z) {
     const double2 u = upos[s];
-    d = cstart[new_cell(Rp, u, A.cpos)] + urank[s];
+    c = new_cell(Rp, u, A.cpos);
+    d = cstart[c] + urank[s];
     if (d < 0 || d >= A.n_total) {
       atomicOr(&A.status->integrity, 1);
       d = -1;
@@ -2069,6 +2072,8 @@ __global__ __launch_bounds__(256) void k_scatter_react(
       if (slot_of) slot_of[p] = d;
       // the pre-reaction types, read by the walk only: models that have one
       if (walk_types) ntype0[d] = t;
+      tw = t;
+      pw = p;
       if ((walk_types >> t) & 1u) walker = true;
       else ntype[d] = nrx > 0 ? react_decide(A, Rp, M, t, p, step, 0ull, 0ull) : t;
     }
@@ -2079,7 +2084,8 @@ __global__ __launch_bounds__(256) void k_scatter_react(
   __syncthreads();
   int base = 0;
   for (int w = 0; w < wv; ++w) base += s_wcnt[w];
-  if (walker) A.wl[blockIdx.x * 256 + base + __popcll(m & ((1ull << lane) - 1ull))] = d;
+  if (walker)
+    A.wl[blockIdx.x * 256 + base + __popcll(m & ((1ull << lane) - 1ull))] = make_int4(d, c, tw, pw);
   if (threadIdx.x == 0) A.wl_n[blockIdx.x] = s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
 }
 
@@ -3269,8 +3275,10 @@ struct rds_ctx {
   int4* d_chunks = nullptr;  // cell-start chunks (replica, first cell, cells, first chunk)
   unsigned long long* d_lb = nullptr;  // [2][n_chunks] look-back words of the scan
   int64_t n_chunks = 0;
-  int* wl = nullptr;         // reaction worklist (cell mode), 256 entries per slot block
+  int4* wl = nullptr;        // reaction worklist, 256 (slot, cell, type, pid) per slot block
   int* wl_n = nullptr;
+  int4* d_wblocks = nullptr;  // (slot block, replica, model) of replicas whose model has a walk
+  int64_t n_wblocks = 0;
   long long* d_step = nullptr;
   DStatus* d_status = nullptr;
   // bonds
@@ -3391,6 +3399,7 @@ StepArgs make_args(rds_ctx* c, const Buf& cur) {
   a.bpart = c->any_bonds ? c->bpart : nullptr;
   a.wl = c->wl;
   a.wl_n = c->wl_n;
+  a.wblocks = c->d_wblocks;
   a.pos4 = cur.pos4;
   a.upos4 = c->upos4;
   a.fbond4 = c->any_bonds ? c->fbond4 : nullptr;
@@ -3653,11 +3662,13 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
       mark(KT_REACT, true);
       const int nb = (int)c->n_sblocks;
       hipLaunchKernelGGL(k_react_list, dim3((unsigned)nb), dim3(256), 0, c->stream, r);
-      const unsigned g = (unsigned)((nb + 3) / 4);  // one wave per slot block
-      if (c->has_mol)
-        hipLaunchKernelGGL((k_react_walk<true>), dim3(g), dim3(256), 0, c->stream, r, nb);
-      else
-        hipLaunchKernelGGL((k_react_walk<false>), dim3(g), dim3(256), 0, c->stream, r, nb);
+      // one wave per slot block of a replica whose model has a walk
+      const int nw = (int)c->n_wblocks;
+      const unsigned g = (unsigned)((nw + 3) / 4);
+      if (nw > 0 && c->has_mol)
+        hipLaunchKernelGGL((k_react_walk<true>), dim3(g), dim3(256), 0, c->stream, r, nw);
+      else if (nw > 0)
+        hipLaunchKernelGGL((k_react_walk<false>), dim3(g), dim3(256), 0, c->stream, r, nw);
       mark(KT_REACT, false);
     }
     c->parity ^= 1;
@@ -3689,12 +3700,13 @@ void enqueue_step(rds_ctx* c, std::vector<std::pair<int, std::pair<hipEvent_t, h
   mark(KT_SCATTER, false);
   if (c->any_rx) {
     mark(KT_REACT, true);
-    const int nb = (int)c->n_sblocks;  // one wave per slot block
-    const unsigned g = (unsigned)((nb + 3) / 4);
-    if (c->has_mol)
-      hipLaunchKernelGGL((k_react_walk<true>), dim3(g), dim3(256), 0, c->stream, r, nb);
-    else
-      hipLaunchKernelGGL((k_react_walk<false>), dim3(g), dim3(256), 0, c->stream, r, nb);
+    // one wave per slot block of a replica whose model has a walk
+    const int nw = (int)c->n_wblocks;
+    const unsigned g = (unsigned)((nw + 3) / 4);
+    if (nw > 0 && c->has_mol)
+      hipLaunchKernelGGL((k_react_walk<true>), dim3(g), dim3(256), 0, c->stream, r, nw);
+    else if (nw > 0)
+      hipLaunchKernelGGL((k_react_walk<false>), dim3(g), dim3(256), 0, c->stream, r, nw);
     mark(KT_REACT, false);
   }
   c->parity ^= 1;
@@ -4000,7 +4012,7 @@ int rds_ctx_destroy(rds_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   std::vector<void*> ptrs = {c->d_rep, c->d_models, c->d_tiles, c->d_rep_of, c->d_sblocks,
                              c->upos, c->ucell, c->urank, c->count, c->cstart, c->d_step,
-                             c->d_chunks, c->d_lb, c->wl, c->wl_n,
+                             c->d_chunks, c->d_lb, c->wl, c->wl_n, c->d_wblocks,
                              c->d_status, c->boff, c->bpart, c->br0, c->bu0, c->fbond,
                              c->slot_of, c->clabel, c->aoff, c->aent, c->apart, c->aprm, c->nl4, c->ncnt, c->xref, c->scell, c->spos,
                              c->stype, c->spid, c->smol};
@@ -4309,6 +4321,17 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
     // reaction worklist: a segment of 256 entries per slot block
     if ((rc = dalloc(&c->wl, sb.size() * 256)) != RDS_OK) return bail(rc);
     if ((rc = dalloc(&c->wl_n, sb.size())) != RDS_OK) return bail(rc);
+    // the reaction walk runs over the slot blocks of replicas whose model has
+    // a catalysed or inhibited reaction only
+    std::vector<int4> wb;
+    for (size_t b = 0; b < sb.size(); ++b) {
+      const int r = sb[b].x, m = c->h_rep[r].model;
+      if (c->h_models[m].walk_types) wb.push_back(make_int4((int)b, r, m, 0));
+    }
+    c->n_wblocks = (int64_t)wb.size();
+    if ((rc = dalloc(&c->d_wblocks, std::max<size_t>(wb.size(), 1))) != RDS_OK) return bail(rc);
+    if (!wb.empty())
+      HIP_TRY(hipMemcpy(c->d_wblocks, wb.data(), sizeof(int4) * wb.size(), hipMemcpyHostToDevice));
 
     HIP_TRY(hipMemcpy(c->d_sblocks, sb.data(), sizeof(int4) * sb.size(), hipMemcpyHostToDevice));
   }
