@@ -2439,11 +2439,10 @@ __device__ __forceinline__ int cell3(const DRep& R, double x, double y, double z
          cell_coord(x, R.cl[0], R.lx);
 }
 
-// f(j) for every slot j of the 27 cells around cell (cx, cy, cz); a
-// reflecting axis keeps to the box, a periodic one wraps (>= 3 cells)
-template <typename F>
-__device__ __forceinline__ void for_stencil3(const DRep& R, const int* __restrict__ cstart,
-                                             int cx, int cy, int cz, F f) {
+// g(c) for each of the 27 cells c around cell (cx, cy, cz); a reflecting
+// axis keeps to the box, a periodic one wraps (>= 3 cells)
+template <typename G>
+__device__ __forceinline__ void for_cells3(const DRep& R, int cx, int cy, int cz, G g) {
   for (int oz = -1; oz <= 1; ++oz) {
     int z = cz + oz;
     if (z < 0 || z >= R.lz) {
@@ -2462,12 +2461,20 @@ __device__ __forceinline__ void for_stencil3(const DRep& R, const int* __restric
           if (!R.sx) continue;
           x = x < 0 ? x + R.lx : x - R.lx;
         }
-        const int c = R.cbase + (z * R.ly + y) * R.lx + x;
-        const int j1 = cstart[c + 1];
-        for (int j = cstart[c]; j < j1; ++j) f(j);
+        g(R.cbase + (z * R.ly + y) * R.lx + x);
       }
     }
   }
+}
+
+// f(j) for every slot j of those cells
+template <typename F>
+__device__ __forceinline__ void for_stencil3(const DRep& R, const int* __restrict__ cstart,
+                                             int cx, int cy, int cz, F f) {
+  for_cells3(R, cx, cy, cz, [&](int c) {
+    const int j1 = cstart[c + 1];
+    for (int j = cstart[c]; j < j1; ++j) f(j);
+  });
 }
 
 // minimum-image displacement x_j - x_i on the periodic axes
@@ -3118,8 +3125,17 @@ __global__ __launch_bounds__(256) void k3_react(const StepArgs A) {
     }
     const double4 xi = A.pos4[s];
     const int mi = MOL ? A.mol[s] : 0;
-    for_stencil3(Rp, A.cstart, cell_coord(xi.x, Rp.cl[0], Rp.lx),
-                 cell_coord(xi.y, Rp.cl[1], Rp.ly), cell_coord(xi.z, Rp.cl[2], Rp.lz), [&](int j) {
+    const int cx = cell_coord(xi.x, Rp.cl[0], Rp.lx), cy = cell_coord(xi.y, Rp.cl[1], Rp.ly);
+    const int cz = cell_coord(xi.z, Rp.cl[2], Rp.lz);
+    // no neighbour count can exceed the particles of the 27 cells: when even
+    // that many could not make the particle react, it keeps its type
+    int pop = 0;
+    for_cells3(Rp, cx, cy, cz, [&](int c) { pop += A.cstart[c + 1] - A.cstart[c]; });
+    if (certainly_inert(A, Rp, M, ti, p, *A.step, pop)) {
+      A.type_out[s] = ti;
+      return;
+    }
+    for_stencil3(Rp, A.cstart, cx, cy, cz, [&](int j) {
       if (j == s) return;
       const int tj = A.type[j];
       if (tj < 0 || tj >= 32 || !((nbr_types >> tj) & 1u)) return;
