@@ -165,14 +165,14 @@ def test_batched_fp64_injected_steps_match_oracle(gpu, ens):
     assert events > 0
 
 
-def _planted(s, steps, seed, k_lin):
+def _planted(s, steps, seed, k_lin, u1_max=2e-4):
     """Injected streams for a wall system whose inactive particles react at
     rate k_lin: at step 1 the uniform of every third inactive particle lies
     halfway between 1 - exp(-dt k) and dt k, where the two forms disagree."""
     n = len(s['x'])
     rng = np.random.Generator(np.random.PCG64(seed))
     normals = rng.standard_normal((steps, n, 2))
-    u1 = rng.random((steps, n)) * 2e-4
+    u1 = rng.random((steps, n)) * u1_max
     u2 = rng.random((steps, n))
     p_lin = s['dt'] * k_lin
     p_exp = -np.expm1(-s['dt'] * k_lin)
@@ -182,11 +182,16 @@ def _planted(s, steps, seed, k_lin):
 
 
 @pytest.mark.parametrize('form', ['linear', 'exp'])
-def test_reaction_probability_forms_match_oracle(gpu, form):
+@pytest.mark.parametrize('u1_max', [2e-4, 5e-3])
+def test_reaction_probability_forms_match_oracle(gpu, form, u1_max):
+    """Both reaction forms against the oracle, event for event.  With u1 up
+    to 5e-3 the active particles' draws (p = dt * k_d * count, at most about
+    dt * k_d * 22 = 2.2e-3 for the 22 or so particles of a stencil) fall below
+    p, between p and that bound, and above it (no walk)."""
     s = small('wall')
     s['types'][::4] = 1
     steps = 12
-    normals, u1, u2, _ = _planted(s, steps, 31, 0.1)
+    normals, u1, u2, _ = _planted(s, steps, 31, 0.1, u1_max)
     eng = engine_for(s, precision='fp64', rng='injected', reaction_prob=form)
     eng.set_injected(normals, u1, u2, first_step=1)
     o = oracle_for(s, rng='injected', reaction_prob=form)
