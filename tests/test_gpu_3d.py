@@ -217,6 +217,56 @@ def test_3d_refuses_what_it_does_not_build(gpu):
         make_replica(0, 10, [5.0], [1], 1e-3, [1.0], [], 1)
 
 
+def test_dense_cluster_3d(gpu):
+    """A dense cubic cluster (8 x 8 x 8 particles 1.05 apart) and a sparse
+    background in a large periodic cube: the LDS capacity follows the mean
+    occupancy, so the tiles that hold the cluster stage it in several LDS
+    rounds.  FP64 and mixed forces against the oracle, then 10 FP64 steps
+    with injected noise: positions within 1e-12, no particle lost."""
+    from pyrdsim_amd import workloads
+    rng = np.random.Generator(np.random.PCG64(5))
+    g = np.arange(8) * 1.05
+    core = np.stack(np.meshgrid(g, g, g), -1).reshape(-1, 3) + 20.0
+    core += rng.uniform(-0.04, 0.04, core.shape)
+    L = 60.0
+    # background: a coarse lattice (spacing 5) away from the cluster
+    b = np.arange(12) * 5.0 + 2.5
+    bg = np.stack(np.meshgrid(b, b, b), -1).reshape(-1, 3)
+    bg = bg[~np.all((bg > 17.0) & (bg < 31.0), axis=1)]
+    x = np.concatenate([core, bg])
+    t = (np.arange(len(x)) % 2).astype(np.intc)
+    s = dict(name='cluster3', x=x, types=t, molecule_ids=workloads._atoms(x, t),
+             L=np.array([L, L, L]), periodic=np.array([1.0, 1.0, 1.0]), dt=5e-5,
+             model=workloads.kalj_model(), bonds=None)
+    o = oracle_for(s)
+    o.init()
+    _, _, a = o.state()
+    scale = force_scale(s) + np.abs(a).max(axis=1)
+    for precision, rtol in (('fp64', 1e-10), ('mixed', 1e-5)):
+        eng = engine_for(s, precision=precision)
+        rows, per, cap = eng.tiling()
+        assert cap < len(core), cap   # the cluster's tiles stage in rounds
+        f = eng.forces()
+        eng.close()
+        err = np.abs(f - a).max(axis=1)
+        bad = (err > rtol * scale) & ~near_cutoff(s)
+        assert not bad.any(), (precision, int(bad.sum()))
+    steps = 10
+    normals, u1, u2 = _injected(len(x), steps, 6)
+    eng = engine_for(s, precision='fp64', rng='injected')
+    eng.set_injected(normals, u1, u2, first_step=1)
+    eng.run(steps)
+    xg, tg = eng.state()
+    eng.close()
+    o = oracle_for(s, rng='injected')
+    o.set_injected(normals, u1, u2, first_step=1)
+    o.init()
+    o.run(1, steps)
+    xo, to, _ = o.state()
+    assert np.abs(_wrap(xg - xo, s)).max() < 1e-12
+    assert np.array_equal(tg, to)
+
+
 @pytest.mark.parametrize('name', ['lj3', 'patchy3'])
 def test_3d_retiling_keeps_state_and_forces(gpu, name):
     """Re-tiling a 3-D context (tiles of 2 x 2 cell rows in y and z, strips
