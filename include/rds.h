@@ -146,7 +146,7 @@ typedef struct {
   int32_t list_capacity;
   /* 2 (0 means 2) or 3.  Three dimensions (pyrdsim.pyx:1159-1185,
    * 1999-2067 loop over N_dim) run the cell search (skin = 0) in FP64 or
-   * mixed precision; the two-dimensional analysis entry points refuse them. */
+   * mixed precision. */
   int32_t n_dim;
   /* harmonic angles (pyrdsim.pyx:1322-1407, 2052-2065) in definition order,
    * global particle indices, host arrays; n_angles may be 0.  angle_j is the

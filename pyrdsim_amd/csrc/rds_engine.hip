@@ -2344,6 +2344,129 @@ __global__ void k_pair_hist(const DRep* __restrict__ rep, const int4* __restrict
     if (s_hist[b]) atomicAdd(&hist[(size_t)sb.x * nbins + b], (unsigned long long)s_hist[b]);
 }
 
+// Three dimensions: f(j) for every slot j of the cells within rad of xi's
+// cell (rings of ceil(rad / cl) cells per axis, the whole axis when the ring
+// would wrap onto itself).
+template <typename F>
+__device__ __forceinline__ void for_cells_within3(const DRep& R, const int* __restrict__ cstart,
+                                                  double4 xi, double rad, F f) {
+  const int cx = cell_coord(xi.x, R.cl[0], R.lx), cy = cell_coord(xi.y, R.cl[1], R.ly);
+  const int cz = cell_coord(xi.z, R.cl[2], R.lz);
+  const int kx = (int)ceil(rad / R.cl[0]), ky = (int)ceil(rad / R.cl[1]);
+  const int kz = (int)ceil(rad / R.cl[2]);
+  const bool fx = 2 * kx + 1 >= R.lx, fy = 2 * ky + 1 >= R.ly, fz = 2 * kz + 1 >= R.lz;
+  const int x0 = fx ? 0 : cx - kx, nx = fx ? R.lx : 2 * kx + 1;
+  const int y0 = fy ? 0 : cy - ky, ny = fy ? R.ly : 2 * ky + 1;
+  const int z0 = fz ? 0 : cz - kz, nz = fz ? R.lz : 2 * kz + 1;
+  for (int oz = 0; oz < nz; ++oz) {
+    int z = z0 + oz;
+    if (z < 0 || z >= R.lz) {
+      if (!R.pz) continue;
+      z = z < 0 ? z + R.lz : z - R.lz;
+    }
+    for (int oy = 0; oy < ny; ++oy) {
+      int y = y0 + oy;
+      if (y < 0 || y >= R.ly) {
+        if (!R.py) continue;
+        y = y < 0 ? y + R.ly : y - R.ly;
+      }
+      for (int ox = 0; ox < nx; ++ox) {
+        int x = x0 + ox;
+        if (x < 0 || x >= R.lx) {
+          if (!R.px) continue;
+          x = x < 0 ? x + R.lx : x - R.lx;
+        }
+        const int c = R.cbase + (z * R.ly + y) * R.lx + x;
+        const int j1 = cstart[c + 1];
+        for (int j = cstart[c]; j < j1; ++j) f(j);
+      }
+    }
+  }
+}
+
+// squared minimum-image distance in three dimensions, each product and sum
+// rounded (no fused multiply-add), as a host computes dx*dx + dy*dy + dz*dz
+__device__ __forceinline__ double dist2_3(const DRep& R, double4 a, double4 b) {
+  double dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+  if (R.px) dx = min_image(dx, R.L[0]);
+  if (R.py) dy = min_image(dy, R.L[1]);
+  if (R.pz) dz = min_image(dz, R.L[2]);
+  return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+// k_pair_hist in three dimensions
+__global__ void k3_pair_hist(const DRep* __restrict__ rep, const int4* __restrict__ sblocks,
+                             const int* __restrict__ cstart, const double4* __restrict__ pos,
+                             const int* __restrict__ type, double r2max, double dr, double rad,
+                             int nbins, unsigned mask, unsigned long long* __restrict__ hist) {
+  extern __shared__ unsigned s_hist[];
+  const int4 sb = sblocks[blockIdx.x];
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x) s_hist[b] = 0u;
+  __syncthreads();
+  const int s = sb.y + (int)threadIdx.x;
+  if (s < sb.z && type_selected(type[s], mask)) {
+    const DRep& R = rep[sb.x];
+    const double4 xi = pos[s];
+    for_cells_within3(R, cstart, xi, rad, [&](int j) {
+      if (j <= s || !type_selected(type[j], mask)) return;
+      const double d2 = dist2_3(R, xi, pos[j]);
+      if (d2 > r2max) return;
+      const int b = (int)floor(sqrt(d2) / dr);
+      if (b < nbins) atomicAdd(&s_hist[b], 1u);
+    });
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += blockDim.x)
+    if (s_hist[b]) atomicAdd(&hist[(size_t)sb.x * nbins + b], (unsigned long long)s_hist[b]);
+}
+
+// Neighbours within r of each particle in three dimensions (r at most a cell
+// length): other particles, or other molecules' in molecular models; out in
+// caller order.
+__global__ void k3_count(const DRep* __restrict__ rep, const int4* __restrict__ sblocks,
+                         const int* __restrict__ cstart, const double4* __restrict__ pos,
+                         const int* __restrict__ mol, const int* __restrict__ pid, double r,
+                         int* __restrict__ out) {
+  const int4 sb = sblocks[blockIdx.x];
+  const int s = sb.y + (int)threadIdx.x;
+  if (s >= sb.z) return;
+  const DRep& R = rep[sb.x];
+  const double4 xi = pos[s];
+  const int mi = mol ? mol[s] : 0;
+  const double r2 = r * r;
+  int cnt = 0;
+  for_cells_within3(R, cstart, xi, r, [&](int j) {
+    const bool other = mol ? mol[j] != mi : j != s;
+    if (other && dist2_3(R, xi, pos[j]) <= r2) ++cnt;
+  });
+  out[pid[s]] = cnt;
+}
+
+// k_cl_prop in three dimensions
+__global__ void k3_cl_prop(const DRep* __restrict__ rep, const int4* __restrict__ sblocks,
+                           const int* __restrict__ cstart, const double4* __restrict__ pos,
+                           double rl2, double rad, int* label, const int* __restrict__ slot_of,
+                           DStatus* status) {
+  const int4 sb = sblocks[blockIdx.x];
+  const int s = sb.y + (int)threadIdx.x;
+  if (s >= sb.z) return;
+  const int l = label[s];
+  if (l < 0) return;
+  const DRep& R = rep[sb.x];
+  const double4 xi = pos[s];
+  int m = l;
+  for_cells_within3(R, cstart, xi, rad, [&](int j) {
+    const int lj = label[j];
+    if (lj < 0 || lj >= m || j == s) return;
+    if (dist2_3(R, xi, pos[j]) <= rl2) m = lj;
+  });
+  if (m < l) {
+    atomicMin(&label[s], m);
+    atomicMin(&label[slot_of[l]], m);  // the particle this one pointed to joins too
+    status->cl_changed = 1;
+  }
+}
+
 __global__ void k_type_counts(const int4* __restrict__ sblocks, const int* __restrict__ type,
                               unsigned long long* __restrict__ counts) {
   __shared__ unsigned s_cnt[MAXT];
@@ -4657,20 +4780,23 @@ int rds_grid3(rds_ctx* c, int32_t* cells_xyz, double* cell_len, int32_t* cell_ba
   return RDS_OK;
 }
 
-// Entry points built for two dimensions only.
-static int two_d_only(rds_ctx* c, const char* what) {
-  if (c->ndim != 2) return fail(RDS_E_ARG, "%s works on two-dimensional systems", what);
-  return RDS_OK;
-}
-
 int rds_neighbor_counts(rds_ctx* c, double r, int32_t* counts) {
   if (!c || !counts) return fail(RDS_E_ARG, "null argument");
-  if (int e = two_d_only(c, "rds_neighbor_counts")) return e;
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   for (int q = 0; q < c->n_rep; ++q) {
     const DRep& d = c->h_rep[q];
-    if (r > std::min(d.cl[0], d.cl[1]))
+    double cmin = std::min(d.cl[0], d.cl[1]);
+    if (c->ndim == 3) cmin = std::min(cmin, d.cl[2]);
+    if (r > cmin)
       return fail(RDS_E_ARG, "probe radius %g exceeds the cell length of replica %d", r, q);
+  }
+  if (c->ndim == 3) {
+    Buf& cur = c->buf[c->parity];
+    hipLaunchKernelGGL(k3_count, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream, c->d_rep,
+                       c->d_sblocks, c->cstart, cur.pos4, c->has_mol ? cur.mol : nullptr, cur.pid,
+                       r, counts);
+    HIP_TRY(hipGetLastError());
+    return RDS_OK;
   }
   // list mode: cells are those of the last rebuild; re-sort for the current positions
   if (c->list) force_rebuild(c);
@@ -4690,8 +4816,8 @@ static int check_radius(rds_ctx* c, double r, const char* what) {
   if (!(r > 0.0) || !std::isfinite(r)) return fail(RDS_E_ARG, "%s must be finite and > 0", what);
   for (int q = 0; q < c->n_rep; ++q) {
     const DRep& d = c->h_rep[q];
-    for (int k = 0; k < 2; ++k)
-      if ((k == 0 ? d.px : d.py) && r > 0.5 * d.L[k])
+    for (int k = 0; k < c->ndim; ++k)
+      if ((k == 0 ? d.px : k == 1 ? d.py : d.pz) && r > 0.5 * d.L[k])
         return fail(RDS_E_ARG, "%s %g exceeds half the box length of replica %d", what, r, q);
   }
   return RDS_OK;
@@ -4700,13 +4826,20 @@ static int check_radius(rds_ctx* c, double r, const char* what) {
 int rds_pair_histogram(rds_ctx* c, double r_max, double dr, int32_t n_bins, uint32_t type_mask,
                        uint64_t* hist) {
   if (!c || !hist) return fail(RDS_E_ARG, "null argument");
-  if (int e = two_d_only(c, "rds_pair_histogram")) return e;
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   if (int e = check_radius(c, r_max, "r_max")) return e;
   if (!(dr > 0.0) || n_bins < 1 || n_bins > 4096)
     return fail(RDS_E_ARG, "need dr > 0 and 1 <= n_bins <= 4096");
   Buf& cur = c->buf[c->parity];
   HIP_TRY(hipMemsetAsync(hist, 0, sizeof(uint64_t) * n_bins * c->n_rep, c->stream));
+  if (c->ndim == 3) {
+    hipLaunchKernelGGL(k3_pair_hist, dim3((unsigned)c->n_sblocks), dim3(256),
+                       sizeof(unsigned) * n_bins, c->stream, c->d_rep, c->d_sblocks, c->cstart,
+                       cur.pos4, cur.type, r_max * r_max, dr, r_max, (int)n_bins,
+                       type_mask ? type_mask : 0xffffffffu, (unsigned long long*)hist);
+    HIP_TRY(hipGetLastError());
+    return RDS_OK;
+  }
   // list mode: cells are those of the last rebuild, each particle within
   // skin/2 of its cell, so the rings reach r_max + skin
   hipLaunchKernelGGL(k_pair_hist, dim3((unsigned)c->n_sblocks), dim3(256),
@@ -4741,9 +4874,14 @@ static int cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t
   for (int pass = 0;; ++pass) {
     if (pass > N) return fail(RDS_E_STATE, "cluster labels did not converge");
     hipLaunchKernelGGL(k_clear_int, dim3(1), dim3(1), 0, c->stream, &c->d_status->cl_changed);
-    hipLaunchKernelGGL(k_cl_prop, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
-                       c->d_rep, c->d_sblocks, c->cstart, cur.pos, c->cpos, r_link * r_link,
-                       r_link + c->skin, lab, c->slot_of, c->d_status);
+    if (c->ndim == 3)
+      hipLaunchKernelGGL(k3_cl_prop, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
+                         c->d_rep, c->d_sblocks, c->cstart, cur.pos4, r_link * r_link, r_link,
+                         lab, c->slot_of, c->d_status);
+    else
+      hipLaunchKernelGGL(k_cl_prop, dim3((unsigned)c->n_sblocks), dim3(256), 0, c->stream,
+                         c->d_rep, c->d_sblocks, c->cstart, cur.pos, c->cpos, r_link * r_link,
+                         r_link + c->skin, lab, c->slot_of, c->d_status);
     hipLaunchKernelGGL(k_cl_jump, dim3(nblk(N, 256)), dim3(256), 0, c->stream, N, lab,
                        c->slot_of);
     HIP_TRY(hipMemcpyAsync(c->h_changed, &c->d_status->cl_changed, sizeof(int),
@@ -4759,7 +4897,6 @@ static int cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t
 
 int rds_cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* labels) {
   if (!c || !labels) return fail(RDS_E_ARG, "null argument");
-  if (int e = two_d_only(c, "rds_cluster_labels")) return e;
   if (!c->loaded) return fail(RDS_E_STATE, "no state loaded");
   if (int e = check_radius(c, r_link, "r_link")) return e;
   if (!c->clabel && dalloc(&c->clabel, c->n_total) != RDS_OK) return RDS_E_HIP;
@@ -4768,7 +4905,6 @@ int rds_cluster_labels(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* l
 
 int rds_cluster_labels_static(rds_ctx* c, double r_link, uint32_t type_mask, int32_t* labels) {
   if (!c || !labels) return fail(RDS_E_ARG, "null argument");
-  if (int e = two_d_only(c, "rds_cluster_labels_static")) return e;
   if (c->n_total > RDS_STATIC_LABELS_MAX)
     return fail(RDS_E_ARG,
                 "rds_cluster_labels_static handles at most %d particles (context has %lld); "

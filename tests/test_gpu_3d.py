@@ -16,7 +16,7 @@ import math
 import numpy as np
 import pytest
 
-from helpers import engine_for, force_scale, near_cutoff, oracle_for, small
+from helpers import brute_pairs, engine_for, force_scale, near_cutoff, oracle_for, small
 
 pytestmark = pytest.mark.gpu
 
@@ -199,22 +199,80 @@ def test_wall3_run_stays_in_the_box(gpu):
 
 def test_3d_refuses_what_it_does_not_build(gpu):
     """Three dimensions run the cell search in FP64 or mixed precision; the
-    list mode, compensated positions and the 2-D analysis entry points are
-    refused with an error, not run."""
+    list mode and compensated positions are refused with an error, not run."""
     s = small('lj3', n=1000)
     with pytest.raises(RuntimeError, match='three dimensions'):
         engine_for(s, skin=0.3)
     with pytest.raises(RuntimeError, match='three dimensions'):
         engine_for(s, precision='compensated')
-    eng = engine_for(s)
-    with pytest.raises(RuntimeError, match='two-dimensional'):
-        eng.neighbor_counts(1.0)
-    with pytest.raises(RuntimeError, match='two-dimensional'):
-        eng.cluster_labels()
     # a 2-D box handed to a 3-D engine is refused by make_replica's caller
     from pyrdsim_amd.engine import make_replica
     with pytest.raises(RuntimeError):
         make_replica(0, 10, [5.0], [1], 1e-3, [1.0], [], 1)
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_analysis_3d_matches_host(gpu, precision):
+    """The device analysis in three dimensions against host brute force on
+    the same state (a batch of an LJ-mixture and a wall replica with active
+    particles, after 30 steps): the pair histogram (all pairs and type 1
+    only) and the RDF from it, neighbour counts within r, type counts and the
+    cluster labels of type 1 linked at 1.5.  Integer results equal except
+    where a pair lies within 1e-9 of a bin edge or of r (flagged)."""
+    from pyrdsim_amd import analysis
+    from pyrdsim_amd.engine import Engine, build_model, make_replica
+    from test_gpu_analysis import _host_hist, _host_labels
+    systems = [small('lj3', n=2500, seed=31), small('wall3', n=2000, seed=32)]
+    systems[1]['types'][::2] = 1
+    models, reps = [], []
+    for k, sy in enumerate(systems):
+        m = sy['model']
+        model, rates = build_model(m['interactions'], m['reactions'], len(m['tau_by_tauD']))
+        models.append(model)
+        reps.append(make_replica(k, len(sy['x']), sy['L'], sy['periodic'], sy['dt'],
+                                 m['tau_by_tauD'], rates, 700 + k))
+    eng = Engine(models, reps, precision=precision, graph_steps=10, n_dim=3)
+    eng.set_state(np.concatenate([sy['x'] for sy in systems]),
+                  np.concatenate([sy['types'] for sy in systems]))
+    eng.run(30)
+    x, t = eng.state()
+    r_max, dr = 4.0, 0.05
+    nb = int(round(r_max / dr))
+    hist = eng.pair_histogram(r_max, dr)
+    hist1 = eng.pair_histogram(r_max, dr, types=(1,))
+    counts = eng.type_counts()
+    labels = eng.cluster_labels(1.5, types=(1,))
+    r_probe = 2.0
+    nbr = eng.neighbor_counts(r_probe)
+    lo = 0
+    for k, sy in enumerate(systems):
+        n = len(sy['x'])
+        xs, ts = x[lo:lo + n], t[lo:lo + n]
+        h, flagged = _host_hist(xs, sy['L'], sy['periodic'], r_max, dr, nb)
+        assert np.abs(hist[k] - h).sum() <= 2 * flagged, (k, np.abs(hist[k] - h).sum())
+        assert hist[k].sum() > 1000
+        act = ts == 1
+        if act.any():
+            h1, f1 = _host_hist(xs[act], sy['L'], sy['periodic'], r_max, dr, nb)
+            assert np.abs(hist1[k] - h1).sum() <= 2 * f1
+        _, g_host = analysis.rdf(xs, sy['L'], sy['periodic'], r_max, dr)
+        _, g_dev = analysis.rdf_from_counts(hist[k], n, sy['L'], r_max, dr, n_dim=3)
+        assert analysis.rdf_error(g_dev, g_host) < 1e-3
+        assert np.array_equal(counts[k][:3], np.bincount(ts, minlength=3)[:3])
+        i, j, d = brute_pairs(xs, sy['L'], sy['periodic'], r_probe + 1e-6)
+        want = np.bincount(i[d <= r_probe], minlength=n) + np.bincount(j[d <= r_probe], minlength=n)
+        near = np.abs(d - r_probe) < 1e-6
+        flag = np.zeros(n, dtype=bool)
+        flag[i[near]] = True
+        flag[j[near]] = True
+        assert np.array_equal(nbr[lo:lo + n][~flag], want[~flag]), k
+        want_l, near_l = _host_labels(xs, ts, sy['L'], sy['periodic'], 1.5, [1])
+        got = labels[lo:lo + n]
+        got = np.where(got >= 0, got - lo, -1)
+        if near_l == 0:
+            assert np.array_equal(got, want_l), (k, int((got != want_l).sum()))
+        lo += n
+    eng.close()
 
 
 def test_dense_cluster_3d(gpu):
