@@ -11,21 +11,27 @@
 // systems ("replicas") are concatenated: replica r owns slots
 // [pbase_r, pbase_r + N_r) and cells [cbase_r, cbase_r + lx_r*ly_r).  Per step:
 //   k_bonds    (molecular models only) harmonic bond and angle forces per slot
-//   k_force    one wavefront per tile (a strip of W cells of one cell row): the
-//              3 x (W+2) neighbour cells are staged in LDS with their periodic
-//              images already shifted; each lane owns one home particle and
-//              loops over its 3x3 stencil in LDS; the force is consumed at once
-//              by the Euler-Maruyama update, the boundary conditions and the
-//              binning of the new position (atomic per-cell histogram + rank)
+//   k_force    one workgroup of R wavefronts per tile of R cell rows by W
+//              cells (home particles dealt 64 at a time): the (R+2) x (W+2) cells are
+//              staged in LDS with their periodic images already shifted; each
+//              lane owns one home particle and walks its three stencil row
+//              windows in LDS; the force is consumed at once by the
+//              Euler-Maruyama update, the boundary conditions and the binning
+//              of the new position (per-cell histogram, one atomic per run of
+//              lanes, and the rank within the cell)
 //   k_cellscan cell starts from the histogram in one pass, in chunks of <= 2048
-//              cells (a replica spreads over many workgroups) with a
-//              decoupled look-back between a replica's chunks; histogram
-//              cleared for the next step, step counter advanced
+//              cells (a replica spreads over many workgroups): each chunk
+//              publishes its count and sums its replica's earlier chunks';
+//              histogram cleared for the next step, step counter advanced
 //   k_scatter  counting-sort reorder of the new state into the other buffer;
 //              with reactions it also decides the particles whose rates need
 //              no neighbours and lists the others
 //   k_react_walk (models with catalysed / inhibited reactions) the listed
-//              particles count their neighbours within r0 and decide
+//              particles count their neighbours within r0 and decide, unless
+//              the particle count of their stencil already shows that no count
+//              could make them react
+// Three dimensions run the same sequence with k3_tile (tiles of 2 x 2 cell
+// rows by W cells, nine row windows per lane), k3_scatter and k3_react.
 // Positions, the minimum-image displacement and the integrator are FP64; the
 // pair-force arithmetic is FP32 (RDS_MIXED) or FP64 (RDS_FP64).
 //
