@@ -2641,19 +2641,33 @@ struct TileSh3 {
   int total, nbatch;
 };
 
-// dynamic LDS of the tiled 3-D kernel: (x, y) as double2, z, type, molecule
+// dynamic LDS of the tiled 3-D kernel.  FP64 pair arithmetic: (x, y) as
+// double2, z, type, molecule.  FP32 pair arithmetic (REL): one float4 per
+// particle, its coordinates relative to the tile's centre rounded once from
+// FP64 and its (shifted) type in the fourth lane's bits, then the molecule.
 struct Staging3 {
   double2* xy;
   double* z;
   int* ty;
+  float4* p4;
   int* mol;
 };
+template <bool REL>
 __device__ __forceinline__ Staging3 staging3(char* smem, int n) {
   Staging3 g;
-  g.xy = (double2*)smem;
-  g.z = (double*)(g.xy + n);
-  g.ty = (int*)(g.z + n);
-  g.mol = g.ty + n;
+  if (REL) {
+    g.p4 = (float4*)smem;
+    g.mol = (int*)(g.p4 + n);
+    g.xy = nullptr;
+    g.z = nullptr;
+    g.ty = nullptr;
+  } else {
+    g.xy = (double2*)smem;
+    g.z = (double*)(g.xy + n);
+    g.ty = (int*)(g.z + n);
+    g.mol = g.ty + n;
+    g.p4 = nullptr;
+  }
   return g;
 }
 
@@ -2789,10 +2803,10 @@ __device__ void tile_setup3(const DRep& Rp, const int* __restrict__ cstart, int 
 }
 
 // Staged indices [p0, p0 + cap) into LDS (types shifted left by TSH).
-template <bool MOL, int TSH>
+template <bool MOL, int TSH, bool REL>
 __device__ void tile_stage3(const TileSh3& sh, int p0, int cap, const double4* __restrict__ pos,
                             const int* __restrict__ type, const int* __restrict__ mol,
-                            const Staging3& G) {
+                            const Staging3& G, double3 org) {
   const int t = threadIdx.x, nthr = blockDim.x;
   const int p1 = min(p0 + cap, sh.total);
   for (int pb = p0 + t; pb < p1; pb += STAGE_U * nthr) {
@@ -2825,9 +2839,16 @@ __device__ void tile_stage3(const TileSh3& sh, int p0, int cap, const double4* _
       const int p = pb + u * nthr;
       if (p >= p1) break;
       const int r = rg[u] / 3, g = rg[u] - 3 * r;
-      G.xy[p - p0] = make_double2(q[u].x + sh.seg_sx[r][g], q[u].y + sh.row_sy[r]);
-      G.z[p - p0] = q[u].z + sh.row_sz[r];
-      G.ty[p - p0] = ty[u] << TSH;
+      const double x = q[u].x + sh.seg_sx[r][g], y = q[u].y + sh.row_sy[r];
+      const double z = q[u].z + sh.row_sz[r];
+      if (REL) {
+        G.p4[p - p0] = make_float4((float)(x - org.x), (float)(y - org.y), (float)(z - org.z),
+                                   __int_as_float(ty[u] << TSH));
+      } else {
+        G.xy[p - p0] = make_double2(x, y);
+        G.z[p - p0] = z;
+        G.ty[p - p0] = ty[u] << TSH;
+      }
       if (MOL) G.mol[p - p0] = mo[u];
     }
   }
@@ -2929,9 +2950,11 @@ __global__ __launch_bounds__(256) void k3_tile(const StepArgs A, int mode, int p
   constexpr bool FAST = KM != 0;
   constexpr int TSH = (FAST && sizeof(Real) == 4) ? 4 : 0;
   constexpr bool LJ6 = TSH && KM == 2;
+  // FP32 pair arithmetic stages coordinates relative to the tile's centre
+  constexpr bool REL = sizeof(Real) == 4;
   extern __shared__ __align__(16) char smem[];
   const int CAPT = A.cap;
-  const Staging3 G = staging3(smem, CAPT + 1);
+  const Staging3 G = staging3<REL>(smem, CAPT + 1);
   __shared__ TileSh3 sh;
   __shared__ int s_pfirst[FAST ? 1 : MAXT * MAXT], s_pcount[FAST ? 1 : MAXT * MAXT];
   __shared__ DTerm s_term[FAST ? 1 : MAXTERMS];
@@ -2978,14 +3001,23 @@ __global__ __launch_bounds__(256) void k3_tile(const StepArgs A, int mode, int p
   tile_setup3(Rp, A.cstart, y0, z0, x0, W, ny, nz, sh);
   const int total = sh.total;
   const bool resident = total <= CAPT;
-  if (resident) {
-    tile_stage3<MOL, TSH>(sh, 0, CAPT, A.pos4, A.type, A.mol, G);
-    if (t == 0) {
-      G.xy[total] = make_double2(0.0, 0.0);
-      G.z[total] = 0.0;
-      G.ty[total] = 0;
-      if (MOL) G.mol[total] = -1;
+  // the centre of the tile's home cells: staged coordinates relative to it
+  // lie within about (W / 2 + 1) cells of it
+  const double3 org = make_double3((x0 + 0.5 * W) * Rp.cl[0], (y0 + 0.5 * ny) * Rp.cl[1],
+                                   (z0 + 0.5 * nz) * Rp.cl[2]);
+  auto sentinel = [&](int e) {
+    if (REL) {
+      G.p4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+      G.xy[e] = make_double2(0.0, 0.0);
+      G.z[e] = 0.0;
+      G.ty[e] = 0;
     }
+    if (MOL) G.mol[e] = -1;
+  };
+  if (resident) {
+    tile_stage3<MOL, TSH, REL>(sh, 0, CAPT, A.pos4, A.type, A.mol, G, org);
+    if (t == 0) sentinel(total);
     __syncthreads();
   }
   const int nc = ny * nz * W;
@@ -3034,6 +3066,10 @@ __global__ __launch_bounds__(256) void k3_tile(const StepArgs A, int mode, int p
     }
     const char* fc_row = reinterpret_cast<const char*>(s_fc4) + ti * 32;
     const int tbase = ti * nt;
+    // the home particle relative to the tile's centre, rounded as its staged
+    // copy is (so that its own difference is exactly zero)
+    const float3 xr = make_float3((float)(xi.x - org.x), (float)(xi.y - org.y),
+                                  (float)(xi.z - org.z));
     Real fx = 0, fy = 0, fz = 0;
     // one candidate at difference (dx, dy, dz), staged type tj (shifted), counted when `live`
     auto pairv = [&](Real dx, Real dy, Real dz, int tj, int mj, bool live) {
@@ -3077,14 +3113,8 @@ __global__ __launch_bounds__(256) void k3_tile(const StepArgs A, int mode, int p
     for (int p0 = 0; p0 < total; p0 += CAPT) {
       if (!resident) {
         __syncthreads();
-        tile_stage3<MOL, TSH>(sh, p0, CAPT, A.pos4, A.type, A.mol, G);
-        if (t == 0) {
-          const int e = min(total - p0, CAPT);
-          G.xy[e] = make_double2(0.0, 0.0);
-          G.z[e] = 0.0;
-          G.ty[e] = 0;
-          if (MOL) G.mol[e] = -1;
-        }
+        tile_stage3<MOL, TSH, REL>(sh, p0, CAPT, A.pos4, A.type, A.mol, G, org);
+        if (t == 0) sentinel(min(total - p0, CAPT));
         __syncthreads();
       }
       if (!valid) continue;
@@ -3098,19 +3128,31 @@ __global__ __launch_bounds__(256) void k3_tile(const StepArgs A, int mode, int p
         const int base = sh.row_base[rr];
         const int lo = max(base + sh.off[rr][k], p0) - p0;
         const int hi = min(base + sh.off[rr][k + 3], p0 + CAPT) - p0;
-        const double2* pq = G.xy + lo;
-        const double2* const pe = G.xy + hi;
-        const double2* const pl = pe - 1;
-        const double* zq = G.z + lo;
-        const int* tq = G.ty + lo;
         const int* mq = G.mol + lo;
-        for (; pq < pe; pq += 2, zq += 2, tq += 2, mq += MOL ? 2 : 0) {
-          const double2 q0 = pq[0], q1 = pq[1];
-          const double z0v = zq[0], z1v = zq[1];
-          const int t0 = tq[0], t1 = tq[1];
-          const int m0 = MOL ? mq[0] : 0, m1 = MOL ? mq[1] : 0;
-          pairv((Real)(q0.x - xi.x), (Real)(q0.y - xi.y), (Real)(z0v - xi.z), t0, m0, true);
-          pairv((Real)(q1.x - xi.x), (Real)(q1.y - xi.y), (Real)(z1v - xi.z), t1, m1, pq < pl);
+        if constexpr (REL) {
+          const float4* pq = G.p4 + lo;
+          const float4* const pe = G.p4 + hi;
+          const float4* const pl = pe - 1;
+          for (; pq < pe; pq += 2, mq += MOL ? 2 : 0) {
+            const float4 q0 = pq[0], q1 = pq[1];
+            const int m0 = MOL ? mq[0] : 0, m1 = MOL ? mq[1] : 0;
+            pairv(q0.x - xr.x, q0.y - xr.y, q0.z - xr.z, __float_as_int(q0.w), m0, true);
+            pairv(q1.x - xr.x, q1.y - xr.y, q1.z - xr.z, __float_as_int(q1.w), m1, pq < pl);
+          }
+        } else {
+          const double2* pq = G.xy + lo;
+          const double2* const pe = G.xy + hi;
+          const double2* const pl = pe - 1;
+          const double* zq = G.z + lo;
+          const int* tq = G.ty + lo;
+          for (; pq < pe; pq += 2, zq += 2, tq += 2, mq += MOL ? 2 : 0) {
+            const double2 q0 = pq[0], q1 = pq[1];
+            const double z0v = zq[0], z1v = zq[1];
+            const int t0 = tq[0], t1 = tq[1];
+            const int m0 = MOL ? mq[0] : 0, m1 = MOL ? mq[1] : 0;
+            pairv((Real)(q0.x - xi.x), (Real)(q0.y - xi.y), (Real)(z0v - xi.z), t0, m0, true);
+            pairv((Real)(q1.x - xi.x), (Real)(q1.y - xi.y), (Real)(z1v - xi.z), t1, m1, pq < pl);
+          }
         }
       }
     }
@@ -4093,7 +4135,9 @@ bool build_tiles3(rds_ctx* c, int target) {
     }
   }
   c->n_tiles = (int64_t)c->h_tiles.size();
-  const int per = 16 + 8 + 4 + (c->has_mol ? 4 : 0);
+  // FP32 pair arithmetic stages one float4 per particle, FP64 a double2, a
+  // double and a type
+  const int per = (c->precision == RDS_FP64 ? 16 + 8 + 4 : 16) + (c->has_mol ? 4 : 0);
   int cap = (int)std::ceil(1.25 * staged_worst / 64.0) * 64;
   cap = std::max(cap, 256);
   cap = std::min(cap, (65536 - 256) / per / 64 * 64);
