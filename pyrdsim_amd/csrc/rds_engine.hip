@@ -4105,8 +4105,13 @@ bool build_tiles(rds_ctx* c, int rows, int target) {
 // occupancy; the LDS round capacity: 1.25x the mean staged count of the
 // densest tile shape, at least 256, at most what 64 KB of dynamic LDS holds
 // (denser tiles stage in rounds).
+// Default particles per row strip of the 3-D tiles (measured on the LJ
+// mixture at density 0.6): wide strips with FP32 pair arithmetic, whose
+// 16-byte staging leaves LDS to spare, narrower ones in FP64.
+int default_target3(const rds_ctx* c) { return c->precision == RDS_FP64 ? 64 : 128; }
+
 bool build_tiles3(rds_ctx* c, int target) {
-  if (target <= 0) target = 48;
+  if (target <= 0) target = default_target3(c);
   c->h_tiles.clear();
   double staged_worst = 0.0;
   int stride = std::max(1, (int)std::lround(0.618 * c->n_rep));
@@ -4175,7 +4180,7 @@ int rds_set_tiling(rds_ctx* c, int32_t tile_rows, int32_t tile_particles) {
     return RDS_E_ARG;
   }
   RDS_TRY(upload_tiles(c));
-  c->target = tile_particles > 0 ? tile_particles : (c->ndim == 3 ? 48 : 56);
+  c->target = tile_particles > 0 ? tile_particles : (c->ndim == 3 ? default_target3(c) : 56);
   if (c->loaded && c->list) {
     // lists hold staged indices of the old tiles
     force_rebuild(c);
@@ -4290,7 +4295,7 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   c->rows = cfg->tile_rows > 0 ? cfg->tile_rows : 1;
   if (c->rows != 1 && c->rows != 2 && c->rows != 4)
     return bail(fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4"));
-  c->target = cfg->tile_particles > 0 ? cfg->tile_particles : (c->ndim == 3 ? 48 : 56);
+  c->target = cfg->tile_particles > 0 ? cfg->tile_particles : (c->ndim == 3 ? default_target3(c) : 56);
   if (cfg->n_replicas >= (1 << 24)) return bail(fail(RDS_E_ARG, "too many replicas"));
   const int spc = cfg->cells_per_cutoff != 0 ? std::abs(cfg->cells_per_cutoff) : 1;
   if (spc != 1) return bail(fail(RDS_E_ARG, "only cells_per_cutoff = 1 (3x3 stencil) is built"));
