@@ -297,7 +297,7 @@ int rds_debug_fill_lds(rds_ctx* ctx, uint32_t pattern);
  * tile_rows cell rows per tile (1, 2, 4) and about tile_particles particles
  * per tile row strip (0: 56 in two dimensions; in three, whose tiles are
  * always 2 x 2 rows in y and z and where tile_rows is checked and reported,
- * not used, 128 with FP32 pair arithmetic and 64 in FP64).  No counterpart in the reference; used by the autotuner.
+ * not used, 128).  No counterpart in the reference; used by the autotuner.
  * rds_get_tiling reports the current choice and LDS capacity. */
 int rds_set_tiling(rds_ctx* ctx, int32_t tile_rows, int32_t tile_particles);
 int rds_get_tiling(rds_ctx* ctx, int32_t* tile_rows, int32_t* tile_particles, int32_t* lds_cap);

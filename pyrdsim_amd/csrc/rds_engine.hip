@@ -2946,7 +2946,7 @@ __device__ __forceinline__ void k3_finish(const StepArgs& A, const DRep& Rp, con
 // as in k_force: 0 = the general term table; 1, 2 = one LJ term per type pair
 // of at most two types (2: exponent n = 6).
 template <typename Real, bool MOL, int KM>
-__global__ __launch_bounds__(256) void k3_tile(const StepArgs A, int mode, int pw) {
+__global__ __launch_bounds__(512) void k3_tile(const StepArgs A, int mode, int pw) {
   constexpr bool FAST = KM != 0;
   constexpr int TSH = (FAST && sizeof(Real) == 4) ? 4 : 0;
   constexpr bool LJ6 = TSH && KM == 2;
@@ -3762,7 +3762,7 @@ void launch_bonds(rds_ctx* c, const Buf& cur) {
 
 template <typename Real, bool MOL>
 void launch_force3_m(rds_ctx* c, const StepArgs& a, int mode) {
-  const dim3 g((unsigned)c->n_tiles), b(256);
+  const dim3 g((unsigned)c->n_tiles), b(512);  // 8 waves per tile
   const size_t sm = c->smem_bytes;
   if (!c->fast)
     hipLaunchKernelGGL((k3_tile<Real, MOL, 0>), g, b, sm, c->stream, a, mode, c->pw);
@@ -4105,10 +4105,10 @@ bool build_tiles(rds_ctx* c, int rows, int target) {
 // occupancy; the LDS round capacity: 1.25x the mean staged count of the
 // densest tile shape, at least 256, at most what 64 KB of dynamic LDS holds
 // (denser tiles stage in rounds).
-// Default particles per row strip of the 3-D tiles (measured on the LJ
-// mixture at density 0.6): wide strips with FP32 pair arithmetic, whose
-// 16-byte staging leaves LDS to spare, narrower ones in FP64.
-int default_target3(const rds_ctx* c) { return c->precision == RDS_FP64 ? 64 : 128; }
+// Default particles per row strip of the 3-D tiles, measured on the LJ
+// mixture at density 0.6 with 8-wave workgroups: about 500 home particles
+// per tile, one per lane.
+int default_target3(const rds_ctx*) { return 128; }
 
 bool build_tiles3(rds_ctx* c, int target) {
   if (target <= 0) target = default_target3(c);
