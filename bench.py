@@ -5,8 +5,9 @@ replicas (256 of the two-species LJ mixture, 256 of the wall + reaction model,
 N per replica log-uniform over 2,048..32,768, sum N = 5,789,872) stepped
 together with FP32 pair forces and FP64 positions, hipGraph-captured.  The
 ensemble is first advanced --pre-advance steps (default 2,000, untimed) from
-its all-inactive start into its reacting regime; the timed window replays
-graphs of at most --steps steps.  Beside it:
+its all-inactive start into its reacting regime, the force kernel's tile
+shape is then tuned on that state (on clones of the engine), and the timed
+window replays graphs of at most --steps steps.  Beside it:
 
 * `plain`: the single-trajectory FP64 path, one replica at a time, on a
   stratified subset of the same replicas, and the ratio batched / plain;
@@ -452,15 +453,17 @@ def main():
         with np.load(args.state, allow_pickle=False) as z:
             x0, t0s, step0 = z['x'], z['t'], int(z['step'])
     eng.set_state(x0, t0s, step0=step0)
+    t = time.perf_counter()
+    advance(eng, args.pre_advance)
+    print('# pre-advanced %d steps in %.1f s' % (args.pre_advance, time.perf_counter() - t),
+          file=sys.stderr, flush=True)
+    # the tile shape is tuned on the state that is then timed (the autotuner
+    # times clones and leaves the run itself where it is)
     tuned = None
     if args.autotune:
         best, times = autotune(eng)
         tuned = dict(best=list(best), force_ms={'%dx%d' % k: round(v, 5) for k, v in times.items()})
         print('# autotune: rows x particles = %s' % (best,), file=sys.stderr, flush=True)
-    t = time.perf_counter()
-    advance(eng, args.pre_advance)
-    print('# pre-advanced %d steps in %.1f s' % (args.pre_advance, time.perf_counter() - t),
-          file=sys.stderr, flush=True)
     st0 = eng.sync()
     secs = time_engine(eng, args.steps, args.warmup)
     st1 = eng.sync()
@@ -572,10 +575,12 @@ def main():
         tsum = 0.0
         plain_tiling = (args.tile_rows, args.tile_particles)
         if args.autotune:
-            # tuned once, on the median-size replica of the subset
+            # tuned once, on the median-size replica of the subset, after the
+            # same pre-advance as the timed runs
             r = pick[len(pick) // 2]
             e1 = Engine([models[reps[r].model]], [ensemble.with_model0(reps[r])], precision='fp64')
             e1.set_state(states[r][0], states[r][1])
+            advance(e1, args.pre_advance)
             plain_tiling, _ = autotune(e1, steps=32)
             e1.close()
         plain_graph = min(args.graph_steps, args.plain_steps)

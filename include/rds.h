@@ -124,7 +124,7 @@ typedef struct {
   int32_t tile_particles;       /* target particles per force tile, 0 = 56 */
   int32_t cells_per_cutoff;     /* 1 -> 3x3 stencil, 2 -> 5x5; 0 = 1 */
   int32_t graph_steps;          /* steps per captured hipGraph, 0 = no graph */
-  int32_t tile_rows;            /* cell rows per force/reaction tile: 1, 2, 4 (0 = 1) */
+  int32_t tile_rows;            /* cell rows per force/reaction tile: 1, 2, 4, 8 (0 = 1) */
   /* harmonic bonds (pyrdsim.pyx:1303-1320, 2037-2047), global particle
    * indices, host arrays; n_bonds may be 0 */
   int64_t n_bonds;
@@ -294,7 +294,7 @@ int rds_time_kernels(rds_ctx* ctx, int64_t n_steps, double* avg_ms, int64_t* cou
 int rds_debug_fill_lds(rds_ctx* ctx, uint32_t pattern);
 
 /* Re-tiles a context (kernel work decomposition only; the state is kept):
- * tile_rows cell rows per tile (1, 2, 4) and about tile_particles particles
+ * tile_rows cell rows per tile (1, 2, 4, 8) and about tile_particles particles
  * per tile row strip (0: 56 in two dimensions; in three, whose tiles are
  * always 2 x 2 rows in y and z and where tile_rows is checked and reported,
  * not used, 128).  No counterpart in the reference; used by the autotuner.

@@ -1,7 +1,8 @@
 """Autotuner for the force kernel's work decomposition.
 
-The tile shape -- cell rows per tile (one wavefront per row, the tile's home
-particles dealt to the wavefronts 64 at a time) and particles per row strip --
+The tile shape -- cell rows per tile (one wavefront per row up to four rows,
+four wavefronts for eight; the tile's home particles dealt to the wavefronts
+64 at a time) and particles per row strip --
 trades LDS staging overhead (bigger tiles stage fewer halo cells per particle)
 against parallelism, LDS occupancy and lane utilisation.  The
 best shape depends on the workload (density, replica sizes, GPU fill), so it is
@@ -16,7 +17,7 @@ import json
 import os
 
 CANDIDATES = ((1, 56), (1, 64), (2, 56), (2, 64), (4, 48), (4, 56), (4, 64), (4, 72), (4, 80),
-              (4, 96), (4, 112), (4, 128))
+              (4, 96), (4, 112), (4, 128), (8, 48), (8, 56), (8, 64))
 
 
 def signature(engine):

@@ -406,8 +406,11 @@ struct Real2 {
 // tiles: R cell rows x W cells.  The staged region is rows cy0-1 .. cy0+R and
 // cells cx0-1 .. cx0+W of each row, periodic images shifted by -L / +L.  Each
 // staged row is at most 3 contiguous slot segments (left image, in-range run,
-// right image).  Wave q of the workgroup owns the home particles of row q.
+// right image).  A tile is served by tile_waves(R) wavefronts (R up to 4, four
+// for taller tiles), which take its home particles 64 at a time.
 // ---------------------------------------------------------------------------
+
+constexpr int tile_waves(int R) { return R < 4 ? R : 4; }
 
 template <int R>
 struct TileSh {
@@ -419,7 +422,7 @@ struct TileSh {
   double row_sy[R + 2];
   int hp[R + 1];             // home particles of the home rows before row q
   int chp[R * MAXW + 1];     // home particles of the home cells before cell i = q * W + k
-  unsigned char hkey[HMAX];  // home number -> its cell (q << 6 | k), when <= HMAX of them
+  unsigned short hkey[HMAX];  // home number -> its cell (q << 6 | k), when <= HMAX of them
   int total, nbatch;
 };
 
@@ -538,12 +541,12 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
   }
   if (t < 64) {
     // first staged index of each staged row and home particles before each
-    // home row: prefix sums over the lanes (NR <= R + 2 <= 6 rows)
+    // home row: prefix sums over the lanes (NR <= R + 2 <= 10 rows)
     const int rt = t < NR ? sh.off[t][W + 2] : 0;
     const int ht = t < nrows ? sh.off[t + 1][W + 1] - sh.off[t + 1][1] : 0;
     int ir = rt, ih = ht;
 #pragma unroll
-    for (int d = 1; d < 8; d <<= 1) {
+    for (int d = 1; d < 16; d <<= 1) {
       const int a = __shfl_up(ir, d, 64), e = __shfl_up(ih, d, 64);
       if (t >= d) {
         ir += a;
@@ -553,8 +556,9 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
     if (t <= NR) sh.row_base[t] = ir - rt;
     if (t == NR) sh.total = ir;
     if (t <= nrows) sh.hp[t] = ih - ht;
-    // the home particles of all rows are dealt to the R waves, 64 per wave
-    if (t == nrows) sh.nbatch = (ih + 64 * R - 1) / (64 * R);
+    // the home particles of all rows are dealt to the workgroup's waves, 64
+    // per wave
+    if (t == nrows) sh.nbatch = (ih + nthr - 1) / nthr;
   }
   // Home cells in row-major order (cell i = q * W + k): the home particles
   // before each cell, and for each home number its cell.
@@ -573,7 +577,7 @@ __device__ void tile_setup(const DRep& Rp, const int* __restrict__ cstart, int c
   if (sh.chp[nc] <= HMAX) {
     for (int c = t; c < nc; c += nthr) {
       const int q = div_small(c, inv_w), k = c - q * W;
-      const unsigned char key = (unsigned char)((q << 6) | k);
+      const unsigned short key = (unsigned short)((q << 6) | k);
       for (int g = sh.chp[c]; g < sh.chp[c + 1]; ++g) sh.hkey[g] = key;
     }
     __syncthreads();
@@ -669,7 +673,7 @@ __device__ __forceinline__ void stage_sentinel(double2* s_pos, int* s_ty, int* s
 // Home particle of lane `lane` of wave wv in batch b, and its three windows.
 // The tile's home particles, taken cell by cell in row-major order of the
 // home cells, are dealt to the waves 64 at a time: home number
-// (b * R + wv) * 64 + lane.  The home cell is the cell whose slot range holds
+// b * blockDim + wv * 64 + lane.  The home cell is the cell whose slot range holds
 // the particle (the cell it was sorted into).
 template <int R>
 struct Home {
@@ -687,7 +691,7 @@ __device__ __forceinline__ void home_windows(const TileSh<R>& sh, int wv, int b,
                                              const double2* __restrict__ pos, Home<R>& h,
                                              double2& xi, int cpos = 0,
                                              const double2* s_home = nullptr) {
-  const int g = (b * R + wv) * 64 + lane;
+  const int g = b * (int)blockDim.x + wv * 64 + lane;
   const int nc = nrows * W;
   const int nh = sh.chp[nc];
   h.valid = g < nh;
@@ -900,13 +904,13 @@ __device__ __forceinline__ bool in_range(double r2, double rc2) {
 #define FORCE_WAVES_LIST 6
 template <typename Real, int KM, int R, bool LIST>
 constexpr int force_waves() {
-  return (KM == 0 || R != 4) ? 1
+  return (KM == 0 || R < 4) ? 1
          : LIST              ? FORCE_WAVES_LIST
                              : (sizeof(Real) == 4 ? FORCE_WAVES32 : FORCE_WAVES64);
 }
 
 template <typename Real, bool MOL, int KM, int R, bool LIST, int PM>
-__global__ __launch_bounds__(64 * R, (force_waves<Real, KM, R, LIST>()))
+__global__ __launch_bounds__(64 * tile_waves(R), (force_waves<Real, KM, R, LIST>()))
 void k_force(const StepArgs A, int mode, int pw) {
   constexpr bool FAST = KM != 0;
   extern __shared__ __align__(16) char smem[];
@@ -1334,7 +1338,7 @@ void k_force(const StepArgs A, int mode, int pw) {
 // Runs only when a rebuild was requested; the last workgroup to finish clears
 // the request.  Grid-stride over tiles, so that an idle launch is short.
 template <bool MOL, int R>
-__global__ __launch_bounds__(64 * R) void k_build(const StepArgs A) {
+__global__ __launch_bounds__(64 * tile_waves(R)) void k_build(const StepArgs A) {
   if (!A.status->rebuild) return;
   extern __shared__ __align__(16) char smem[];
   const int CAPT = A.cap;
@@ -3499,7 +3503,7 @@ struct rds_ctx {
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   int graph_len = 0;
   // kernel variant
-  int rows = 1;         // cell rows per tile (1, 2 or 4)
+  int rows = 1;         // cell rows per tile (1, 2, 4 or 8)
   bool fast = false;    // all models: <= 2 types, one LJ term per pair, same even n
   int pw = 0;           // n / 2 of that LJ exponent
   int cap = 384;        // staged particles per LDS round
@@ -3598,6 +3602,7 @@ void launch_force_r(rds_ctx* c, const StepArgs& a, int mode) {
   dim3 g((unsigned)c->n_tiles);
   const size_t sm = c->smem_bytes;
   switch (c->rows) {
+    case 8: hipLaunchKernelGGL((k_force<Real, MOL, KM, 8, LIST, PM>), g, dim3(256), sm, c->stream, a, mode, c->pw); break;
     case 4: hipLaunchKernelGGL((k_force<Real, MOL, KM, 4, LIST, PM>), g, dim3(256), sm, c->stream, a, mode, c->pw); break;
     case 2: hipLaunchKernelGGL((k_force<Real, MOL, KM, 2, LIST, PM>), g, dim3(128), sm, c->stream, a, mode, c->pw); break;
     default: hipLaunchKernelGGL((k_force<Real, MOL, KM, 1, LIST, PM>), g, dim3(64), sm, c->stream, a, mode, c->pw); break;
@@ -3651,6 +3656,7 @@ void launch_build(rds_ctx* c, const StepArgs& a) {
   const unsigned g = (unsigned)std::min<int64_t>(c->n_tiles, 4096);
   const size_t sm = c->smem_bytes;
   switch (c->rows) {
+    case 8: hipLaunchKernelGGL((k_build<MOL, 8>), dim3(g), dim3(256), sm, c->stream, a); break;
     case 4: hipLaunchKernelGGL((k_build<MOL, 4>), dim3(g), dim3(256), sm, c->stream, a); break;
     case 2: hipLaunchKernelGGL((k_build<MOL, 2>), dim3(g), dim3(128), sm, c->stream, a); break;
     default: hipLaunchKernelGGL((k_build<MOL, 1>), dim3(g), dim3(64), sm, c->stream, a); break;
@@ -4047,8 +4053,8 @@ int fill_model(const rds_model& m, DModel& d, double* rcut, int ndim) {
 // capacity: 1.5x the mean staged count of the densest tile shape, at least
 // 256, at most what 64 KB of dynamic LDS holds (denser tiles stage in rounds).
 bool build_tiles(rds_ctx* c, int rows, int target) {
-  if (rows != 1 && rows != 2 && rows != 4) {
-    fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4");
+  if (rows != 1 && rows != 2 && rows != 4 && rows != 8) {
+    fail(RDS_E_ARG, "tile_rows must be 1, 2, 4 or 8");
     return false;
   }
   if (target <= 0) target = 56;
@@ -4084,11 +4090,14 @@ bool build_tiles(rds_ctx* c, int rows, int target) {
   const int per = 16 + 4 + (c->has_mol ? 4 : 0);
   // 1.5x headroom, trimmed (not below 1.15x) where it would cost the force
   // kernel occupancy: 4 x waves-per-SIMD waves per CU of the register-resident
-  // path leave 160 KB / (4 waves / rows) of LDS per workgroup
-  const size_t tile_sh = rows == 4 ? sizeof(TileSh<4>) : rows == 2 ? sizeof(TileSh<2>)
-                                                                   : sizeof(TileSh<1>);
+  // path leave 160 KB / (4 waves / tile waves) of LDS per workgroup
+  const size_t tile_sh = rows == 8   ? sizeof(TileSh<8>)
+                         : rows == 4 ? sizeof(TileSh<4>)
+                         : rows == 2 ? sizeof(TileSh<2>)
+                                     : sizeof(TileSh<1>);
   const int waves = c->precision == RDS_FP64 ? FORCE_WAVES64 : FORCE_WAVES32;
-  const long budget = (long)(160 * 1024 * rows / (4 * waves)) - (long)tile_sh - 64;
+  const long budget =
+      (long)(160 * 1024 * tile_waves(rows) / (4 * waves)) - (long)tile_sh - 64;
   const int cap_occ = budget > 0 ? (int)(budget / per / 64 * 64) : 0;
   int cap = (int)std::ceil(1.5 * staged_worst / 64.0) * 64;
   cap = std::max((int)std::ceil(1.15 * staged_worst / 64.0) * 64, std::min(cap, cap_occ));
@@ -4172,8 +4181,8 @@ int rds_set_tiling(rds_ctx* c, int32_t tile_rows, int32_t tile_particles) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (c->ndim == 3) {
     // tiles of T3Y x T3Z rows; tile_rows is accepted and kept, not used
-    if (tile_rows != 1 && tile_rows != 2 && tile_rows != 4)
-      return fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4");
+    if (tile_rows != 1 && tile_rows != 2 && tile_rows != 4 && tile_rows != 8)
+      return fail(RDS_E_ARG, "tile_rows must be 1, 2, 4 or 8");
     c->rows = tile_rows;
     build_tiles3(c, tile_particles);
   } else if (!build_tiles(c, tile_rows, tile_particles)) {
@@ -4293,8 +4302,8 @@ int rds_ctx_create(const rds_config* cfg, rds_ctx** out) {
   c->n_rep = cfg->n_replicas;
   c->n_models = cfg->n_models;
   c->rows = cfg->tile_rows > 0 ? cfg->tile_rows : 1;
-  if (c->rows != 1 && c->rows != 2 && c->rows != 4)
-    return bail(fail(RDS_E_ARG, "tile_rows must be 1, 2 or 4"));
+  if (c->rows != 1 && c->rows != 2 && c->rows != 4 && c->rows != 8)
+    return bail(fail(RDS_E_ARG, "tile_rows must be 1, 2, 4 or 8"));
   c->target = cfg->tile_particles > 0 ? cfg->tile_particles : (c->ndim == 3 ? default_target3(c) : 56);
   if (cfg->n_replicas >= (1 << 24)) return bail(fail(RDS_E_ARG, "too many replicas"));
   const int spc = cfg->cells_per_cutoff != 0 ? std::abs(cfg->cells_per_cutoff) : 1;

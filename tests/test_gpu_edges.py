@@ -76,6 +76,7 @@ def test_periodic_box_below_three_cells_is_refused(gpu):
 
 
 @pytest.mark.parametrize('kw', [dict(), dict(tile_rows=4, tile_particles=144),
+                                dict(tile_rows=8, tile_particles=64),
                                 dict(skin=0.3, list_capacity=8)])
 def test_dense_cluster_in_large_box(gpu, kw):
     """400 particles on a jittered square lattice of spacing 1.05 (density

@@ -103,7 +103,7 @@ def test_list_overflow_falls_back_to_stencil(gpu, name):
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize('rows,per_tile', [(1, 8), (2, 56), (4, 400), (1, 400)])
+@pytest.mark.parametrize('rows,per_tile', [(1, 8), (2, 56), (4, 400), (1, 400), (8, 64)])
 @pytest.mark.parametrize('name', ['wall', 'patchy'])
 def test_list_tile_shapes_and_lds_rounds(gpu, name, rows, per_tile):
     """List mode equals the cell mode for every tile shape, including staged

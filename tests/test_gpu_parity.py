@@ -233,7 +233,8 @@ def test_out_of_bounds_raises(gpu):
         eng.sync()
 
 
-@pytest.mark.parametrize('rows,per_tile', [(1, 8), (2, 20), (4, 56), (1, 400), (4, 400)])
+@pytest.mark.parametrize('rows,per_tile', [(1, 8), (2, 20), (4, 56), (1, 400), (4, 400),
+                                           (8, 24), (8, 64), (8, 400)])
 @pytest.mark.parametrize('name', ['kalj', 'wall', 'patchy'])
 def test_tile_shapes(gpu, name, rows, per_tile):
     """Every tile shape gives the same forces and the same injected-noise
@@ -267,7 +268,7 @@ def test_tile_shapes(gpu, name, rows, per_tile):
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize('rows,per_tile', [(1, 56), (2, 56), (4, 112)])
+@pytest.mark.parametrize('rows,per_tile', [(1, 56), (2, 56), (4, 112), (8, 64)])
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 def test_no_reads_of_unstaged_lds(gpu, rows, per_tile, precision):
     """Regression: with the LDS of the GPU filled with NaN bits before the
