@@ -49,8 +49,8 @@ LIST_BYTES_PER_PARTICLE = 62
 LIST_BYTES_PER_ENTRY = 2
 # the force-only figure of the pair-force step (read pos 16 + type 4, write the
 # force 8 per particle; 8 per cell), for comparison with the fused count above
-SURVEY_FORCE_BYTES_PER_PARTICLE = 28
-SURVEY_FORCE_BYTES_PER_CELL = 8
+FORCE_ONLY_BYTES_PER_PARTICLE = 28
+FORCE_ONLY_BYTES_PER_CELL = 8
 
 
 def time_engine(eng, steps, warmup):
@@ -535,8 +535,8 @@ def main():
                     algorithmic_bytes=int(force_bytes),
                     copy_measured=round(copy_gbs, 1),
                     achieved_force_only_bytes=round(
-                        (SURVEY_FORCE_BYTES_PER_PARTICLE * n_tot +
-                         SURVEY_FORCE_BYTES_PER_CELL * n_cells) / (force_ms * 1e-3) / 1e9, 1),
+                        (FORCE_ONLY_BYTES_PER_PARTICLE * n_tot +
+                         FORCE_ONLY_BYTES_PER_CELL * n_cells) / (force_ms * 1e-3) / 1e9, 1),
                     frac_of_measured_copy=round(achieved / copy_gbs, 4),
                     avg_ms={k: round(v[0], 5) for k, v in kt.items() if v[1]},
                     at_step=int(args.pre_advance + args.warmup + args.steps +
